@@ -1,0 +1,185 @@
+#!/usr/bin/env python3
+"""Benchmark: sparse D3Q19 MRT on a 256^3 porous sphere pack, one MI355X.
+
+Workload: 256^3 uint8 geometry from a seeded overlapping-sphere pack (radii
+uniform in [6,14], spheres added until porosity <= 0.35, periodic in y and z),
+fixed pressure on the x faces (rho 1.0005 left, 1.0 right), y/z periodic,
+niu = 0.1, FP32 populations.  A step is one fused collide+stream launch over
+the fluid list.  Prints one JSON line.
+
+  python bench.py [--gpus 1] [--steps K] [--warmup W] [--dtype f32|f16|bf16]
+                  [--no-steady] [--no-cpu-baseline]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0           # MI355X HBM3E, GB/s
+BYTES_PER_FLUP = {"f32": 152, "f16": 76, "bf16": 76}  # 19 reads + 19 writes per fluid-cell update
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "step_traffic_pack256.json")
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def make_geometry(n, seed):
+    from taichi_lbm3d_amd.geometry import sphere_pack
+    t0 = time.time()
+    solid = sphere_pack((n, n, n), porosity=0.35, rmin=6.0, rmax=14.0, seed=seed, periodic=(False, True, True))
+    log(f"geometry {n}^3 seed {seed}: porosity {(solid == 0).mean():.4f} ({time.time() - t0:.1f}s)")
+    return solid
+
+
+def make_solver(solid, dtype):
+    from taichi_lbm3d_amd import LB3D_Solver_Single_Phase
+    n = solid.shape
+    s = LB3D_Solver_Single_Phase(*n, sparse_storage=True, dtype=dtype)
+    s.solid.from_numpy(solid)
+    s.niu = 0.1
+    s.set_bc_rho_x0(1.0005)
+    s.set_bc_rho_x1(1.0)
+    s.init_simulation()
+    return s
+
+
+def cpu_baseline(solid, budget_s=15.0):
+    """The C restatement of the reference's four-kernel step (sparse storage,
+    FP32, OpenMP over this host's cores), timed on a bounded step count."""
+    import oracle
+    from oracle import COracle
+    oracle.build()
+    threads = len(os.sched_getaffinity(0))
+    n = solid.shape
+    o = COracle(*n, sparse_storage=True, real="f32", threads=threads)
+    o.set_solid(solid)
+    o.niu = 0.1
+    o.set_bc_rho("x_left", 1.0005)
+    o.set_bc_rho("x_right", 1.0)
+    o.init_simulation()
+    n_fluid = int((solid == 0).sum())
+    o.step(1)  # warm-up
+    t0 = time.perf_counter()
+    o.step(1)
+    one = time.perf_counter() - t0
+    steps = max(2, min(200, int(budget_s / max(one, 1e-6))))
+    t0 = time.perf_counter()
+    o.step(steps)
+    dt = time.perf_counter() - t0
+    cpu_model = ""
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                cpu_model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return {"value": n_fluid * steps / dt / 1e6, "unit": "MLUPS", "cores": threads, "kind": "port",
+            "sample": f"{steps} steps of the {n[0]}^3 pack after 1 warm-up step, {dt:.1f}s wall; C/OpenMP "
+                      f"restatement of the reference's colission/streaming1/Boundary_condition/streaming3 "
+                      f"over active 3^3 blocks (not Taichi); CPU {cpu_model}"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2000)
+    ap.add_argument("--warmup", type=int, default=200)
+    ap.add_argument("--n", type=int, default=256)
+    ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--dtype", default="f32", choices=["f32", "f16", "bf16"])
+    ap.add_argument("--no-steady", action="store_true")
+    ap.add_argument("--max-steady-steps", type=int, default=50000)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only: --gpus {args.gpus} is not supported (the solver is single-GPU; "
+              "see the sharding design for the multi-GPU layout)", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    assert torch.cuda.is_available(), "bench.py needs a GPU"
+    torch.cuda.set_device(0)
+    solid = make_geometry(args.n, args.seed)
+    s = make_solver(solid, args.dtype)
+    n_fluid = s.n_fluid
+    log(f"fluid cells {n_fluid}, interior {s._params.n_interior}")
+
+    s.run(args.warmup)
+    torch.cuda.synchronize()
+    start, end = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    start.record()
+    s.run(args.steps)
+    end.record()
+    torch.cuda.synchronize()
+    ms = start.elapsed_time(end)
+    ms_per_step = ms / args.steps
+    mlups = n_fluid * args.steps / (ms / 1e3) / 1e6
+    r = s.reduce()
+    if r["nonfinite"]:
+        raise FloatingPointError("non-finite fields after the timed run")
+    bpf = BYTES_PER_FLUP[args.dtype]
+    achieved = n_fluid * bpf / (ms_per_step / 1e3) / 1e9
+    traffic = None
+    if os.path.exists(TRAFFIC_FILE) and args.n == 256 and args.dtype == "f32":
+        with open(TRAFFIC_FILE) as fh:
+            traffic = json.load(fh).get("bytes_per_launch")
+    log(f"timed {args.steps} steps: {ms:.1f} ms, {mlups:.0f} MLUPS, {achieved:.0f} GB/s algorithmic")
+
+    steady = None
+    if not args.no_steady:
+        s2 = make_solver(solid, args.dtype)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        steps, ok, hist = s2.run_to_steady(tol=1e-6, interval=500, max_steps=args.max_steady_steps)
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        k = s2.permeability()
+        steady = {"steps": steps, "converged": ok, "wall_s": round(wall, 3), "permeability_lu2": k,
+                  "mean_vx": hist[-1], "criterion": "|d<vx>|/|<vx>| < 1e-6 over 500 steps, max "
+                  f"{args.max_steady_steps}"}
+        log(f"steady: {steps} steps converged={ok} {wall:.2f}s k={k:.5g}")
+        del s2
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        cpu = cpu_baseline(solid)
+        log(f"cpu baseline: {cpu['value']:.1f} MLUPS on {cpu['cores']} cores")
+
+    out = {
+        "metric": "MLUPS + % HBM roofline, D3Q19 MRT 256^3 porous; time-to-steady permeability",
+        "value": round(mlups, 1),
+        "unit": "MLUPS",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 5),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": args.dtype,
+        "data": "synthetic",
+        "config": {"workload": f"sparse D3Q19 MRT, {args.n}^3 overlapping-sphere pack (seed {args.seed}, radii "
+                               f"6-14, porosity {float((solid == 0).mean()):.4f}), pressure x faces 1.0005/1.0, "
+                               f"y/z periodic, niu 0.1, reference convention",
+                   "fluid_cells": n_fluid, "storage": args.dtype, "parallelism": "single GPU"},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                     "bytes_per_flup": bpf},
+        "cpu_baseline": cpu,
+        "steady": steady,
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,154 @@
+/*
+ * lbm3d.h -- C ABI of liblbm3d.so, the MI355X (gfx950) single-phase sparse
+ * D3Q19 MRT lattice-Boltzmann path.
+ *
+ * The reference exposes this path as the Python class LB3D_Solver_Single_Phase
+ * (Single_phase/LBM_3D_SinglePhase_Solver.py, "SP" below) whose Taichi
+ * kernels run on the Taichi runtime.  These entry points replace those
+ * kernels; the Python host class in taichi_lbm3d_amd/solver.py keeps the
+ * reference's surface and calls them through ctypes.
+ *
+ * Conventions shared by every entry point:
+ *   - All pointers except `p` are device pointers owned by the caller; the
+ *     library never allocates or frees memory and keeps no pointer between calls.
+ *   - `stream` is a hipStream_t (NULL = the default stream).  Work is only
+ *     enqueued; nothing synchronises unless stated.
+ *   - Return 0 on success, a negative LBM3D_E* code otherwise;
+ *     lbm3d_last_error() then gives a message (per thread).
+ *   - Populations are stored shifted, g_q = f_q - w_q, in SoA order
+ *     g[q * n_cells + cell], q = 0..18 in the reference's direction order
+ *     (SP:183-187), in the storage type `dtype` (LBM3D_F32/F16/BF16);
+ *     arithmetic is FP32.
+ *   - Fluid cells are numbered: interior cells first, then cells owned by a
+ *     pressure/velocity face, each group in C order (i, j, k; k fastest).
+ */
+#ifndef LBM3D_H
+#define LBM3D_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define LBM3D_ABI_VERSION 1
+
+/* storage types for populations */
+#define LBM3D_F32 0
+#define LBM3D_F16 1
+#define LBM3D_BF16 2
+
+/* error codes */
+#define LBM3D_OK 0
+#define LBM3D_EINVAL (-1) /* bad argument (null pointer, size, mode) */
+#define LBM3D_EHIP (-2)   /* a HIP runtime call failed */
+#define LBM3D_ESIZE (-3)  /* a size exceeds what the kernels index */
+
+/* Face order everywhere: 0 x-left, 1 x-right, 2 y-left, 3 y-right, 4 z-left,
+ * 5 z-right -- the order in which SP's Boundary_condition applies them
+ * (SP:274-370), so a node on several faces belongs to the last one. */
+
+/* Frozen solver parameters: what SP bakes into its kernels at
+ * init_simulation() and first compilation (SP:118-149, SP:230, SP:274-365). */
+typedef struct lbm3d_params {
+  int32_t nx, ny, nz;  /* grid extent (SP:11) */
+  int32_t n_cells;     /* fluid cells after compaction */
+  int32_t n_interior;  /* cells [0, n_interior) are interior, the rest face-owned */
+  int32_t force_on;    /* force_flag (SP:137-140) */
+  float s_v, s_o;      /* relaxation rates s_v and s_other (SP:127-129) */
+  float fx, fy, fz;    /* body force (SP:17, SP:134-136) */
+  float guo_a, guo_b;  /* force-term scalings: SP uses 1/3 and 1/9 (SP:236) */
+  int32_t bc_mode[6];  /* 0 periodic, 1 fixed pressure, 2 fixed velocity (SP:22) */
+  float rho_bc[6];     /* rho_bcxl ... (SP:23-28) */
+  float u_bc[6][3];    /* vx_bcxl, vy_bcxl, vz_bcxl ... (SP:23-28) */
+} lbm3d_params;
+
+/* Face tag of a face-owned cell: bits 0-2 owner face, bits 3-4 mode, bit 5
+ * set when the owner's inward neighbour is solid (SP:278, :294, ...). */
+#define LBM3D_TAG_FACE(t) ((t) & 7)
+#define LBM3D_TAG_MODE(t) (((t) >> 3) & 3)
+#define LBM3D_TAG_INWARD_SOLID(t) (((t) >> 5) & 1)
+
+int lbm3d_abi_version(void);
+const char *lbm3d_last_error(void);
+
+/* ---- geometry: the compacted fluid list (replaces SP's pointer/dense SNode
+ * tree, SP:41-44, and the fluid test at SP:225, :262, :376) ---------------- */
+
+/* Workspace bytes that lbm3d_compact_count / lbm3d_compact_fill need for a
+ * grid of n_dense = nx*ny*nz cells. */
+size_t lbm3d_compact_workspace_bytes(int64_t n_dense);
+
+/* Pass 1: classify every cell of `solid` (uint8 C-order (nx,ny,nz), nonzero =
+ * solid, SP:173-177) as solid / interior fluid / face-owned fluid (owner per
+ * the bc_mode of `p`), and write counts[0] = interior cells, counts[1] =
+ * face-owned cells (device int32[2]).  Fields of `p` used: nx, ny, nz, bc_mode. */
+int lbm3d_compact_count(void *stream, const lbm3d_params *p, const uint8_t *solid,
+                        int32_t *counts, void *workspace, size_t workspace_bytes);
+
+/* Pass 2 (after pass 1 with the same workspace): number the fluid cells.
+ * Writes dense_to_compact[n_dense] (compact id or -1 for solid), cells[n_cells]
+ * (C-order dense index of each fluid cell), face_tag[n_cells - n_interior].
+ * p->n_interior and p->n_cells must hold the counts of pass 1. */
+int lbm3d_compact_fill(void *stream, const lbm3d_params *p, const uint8_t *solid,
+                       int32_t *dense_to_compact, int32_t *cells, int32_t *face_tag,
+                       void *workspace, size_t workspace_bytes);
+
+/* Neighbour table, SoA nbr[(q-1) * n_cells + cell] for q = 1..18: the compact
+ * id of the cell at (i,j,k)+e_q with the unconditional periodic wrap of SP's
+ * periodic_index (SP:247-257), or -1 when that cell is solid, i.e. when
+ * streaming1 bounces f_q back into the cell's own opposite slot (SP:265-268). */
+int lbm3d_build_neighbors(void *stream, const lbm3d_params *p, const int32_t *dense_to_compact,
+                          const int32_t *cells, int32_t *nbr);
+
+/* ---- solver ------------------------------------------------------------- */
+
+/* init (SP:160-170): g = feq(rho0, u0) - w on every fluid cell, face_u = u0 on
+ * face-owned cells.  rho0 [n_cells] and u0 (SoA [3][n_cells]) may be NULL for
+ * the reference's rest state rho = 1, u = 0.  face_u is SoA [3][n_face]. */
+int lbm3d_init(void *stream, const lbm3d_params *p, int dtype, void *g, float *face_u,
+               const float *rho0, const float *u0);
+
+/* One time step: the reference's colission -> streaming1 ->
+ * Boundary_condition -> streaming3 (SP:477-481) as one fused kernel over the
+ * fluid list, in-place with the AA access pattern.  `t` is the number of steps
+ * already taken; its parity selects the access pattern, and t == 0 is the
+ * step that starts from init's state (macros rho0, u0 without the half-force
+ * shift, face populations as initialised). */
+int lbm3d_step(void *stream, const lbm3d_params *p, int dtype, void *g, const int32_t *nbr,
+               const int32_t *face_tag, float *face_u, int64_t t);
+
+/* `nsteps` consecutive lbm3d_step calls from step t0, enqueued back to back. */
+int lbm3d_run(void *stream, const lbm3d_params *p, int dtype, void *g, const int32_t *nbr,
+              const int32_t *face_tag, float *face_u, int64_t t0, int32_t nsteps);
+
+/* Macroscopic fields after t steps, what SP's streaming3 leaves in rho and v
+ * (SP:376-388): rho[n_cells] and v (SoA [3][n_cells]), FP32. */
+int lbm3d_macros(void *stream, const lbm3d_params *p, int dtype, const void *g,
+                 const int32_t *nbr, const int32_t *face_tag, const float *face_u, int64_t t,
+                 float *rho, float *v);
+
+/* Scatter compact rho/v into dense C-order arrays of extent (ox,oy,oz) >=
+ * (nx,ny,nz): rho_dense[ox*oy*oz], v_dense[ox*oy*oz][3] (the to_numpy layout).
+ * Cells that are not fluid are left untouched. */
+int lbm3d_scatter_dense(void *stream, const lbm3d_params *p, const int32_t *cells,
+                        const float *rho, const float *v, int32_t ox, int32_t oy, int32_t oz,
+                        float *rho_dense, float *v_dense);
+
+/* FP64 reductions over the compact fields (cal_max_v, SP:399-402, plus the
+ * flux sums used for permeability and convergence):
+ *   out[0] sum rho        out[1..3] sum v       out[4] max |v|
+ *   out[5] count of non-finite rho or v         out[6] sum vx over i_lo <= i <= i_hi
+ *   out[7] number of cells in that slab range
+ * `out` is device double[8]; workspace of lbm3d_reduce_workspace_bytes(). */
+size_t lbm3d_reduce_workspace_bytes(int32_t n_cells);
+int lbm3d_reduce(void *stream, const lbm3d_params *p, const int32_t *cells, const float *rho,
+                 const float *v, int32_t i_lo, int32_t i_hi, double *out, void *workspace,
+                 size_t workspace_bytes);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* LBM3D_H */

@@ -1,0 +1,119 @@
+"""ctypes binding of liblbm3d.so (the C ABI declared in include/lbm3d.h).
+
+The library is built in-tree by :func:`build` (hipcc, gfx950).  Loading fails
+loudly when it is missing: there is no CPU fallback for the solver.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_ROOT = os.path.dirname(_HERE)
+LIB_PATH = os.path.join(_HERE, "liblbm3d.so")
+SOURCES = [os.path.join(_HERE, "csrc", "lbm3d.hip")]
+INCLUDE = os.path.join(_ROOT, "include")
+
+ABI_VERSION = 1
+F32, F16, BF16 = 0, 1, 2
+DTYPES = {"f32": F32, "f16": F16, "bf16": BF16}
+HIPCC_FLAGS = [
+    "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",
+    # keep FP32 arithmetic unpacked (no v_pk_*_f32)
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+]
+
+
+def build(verbose: bool = False) -> str:
+    """Compile liblbm3d.so for gfx950 with hipcc; returns its path."""
+    hipcc = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+    tmp = LIB_PATH + ".tmp"
+    cmd = [hipcc, *HIPCC_FLAGS, "-I", INCLUDE, "-o", tmp, *SOURCES]
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    if res.returncode != 0:
+        raise RuntimeError("hipcc failed:\n" + res.stdout + res.stderr)
+    if verbose:
+        print(" ".join(cmd))
+    os.replace(tmp, LIB_PATH)
+    return LIB_PATH
+
+
+class Params(ctypes.Structure):
+    """Mirror of struct lbm3d_params."""
+    _fields_ = [
+        ("nx", ctypes.c_int32), ("ny", ctypes.c_int32), ("nz", ctypes.c_int32),
+        ("n_cells", ctypes.c_int32), ("n_interior", ctypes.c_int32), ("force_on", ctypes.c_int32),
+        ("s_v", ctypes.c_float), ("s_o", ctypes.c_float),
+        ("fx", ctypes.c_float), ("fy", ctypes.c_float), ("fz", ctypes.c_float),
+        ("guo_a", ctypes.c_float), ("guo_b", ctypes.c_float),
+        ("bc_mode", ctypes.c_int32 * 6),
+        ("rho_bc", ctypes.c_float * 6),
+        ("u_bc", (ctypes.c_float * 3) * 6),
+    ]
+
+
+# name -> (restype, argtypes); every symbol include/lbm3d.h declares
+_vp, _i32, _i64, _sz = ctypes.c_void_p, ctypes.c_int32, ctypes.c_int64, ctypes.c_size_t
+_pp = ctypes.POINTER(Params)
+SIGNATURES = {
+    "lbm3d_abi_version": (ctypes.c_int, []),
+    "lbm3d_last_error": (ctypes.c_char_p, []),
+    "lbm3d_compact_workspace_bytes": (_sz, [_i64]),
+    "lbm3d_compact_count": (ctypes.c_int, [_vp, _pp, _vp, _vp, _vp, _sz]),
+    "lbm3d_compact_fill": (ctypes.c_int, [_vp, _pp, _vp, _vp, _vp, _vp, _vp, _sz]),
+    "lbm3d_build_neighbors": (ctypes.c_int, [_vp, _pp, _vp, _vp, _vp]),
+    "lbm3d_init": (ctypes.c_int, [_vp, _pp, ctypes.c_int, _vp, _vp, _vp, _vp]),
+    "lbm3d_step": (ctypes.c_int, [_vp, _pp, ctypes.c_int, _vp, _vp, _vp, _vp, _i64]),
+    "lbm3d_run": (ctypes.c_int, [_vp, _pp, ctypes.c_int, _vp, _vp, _vp, _vp, _i64, _i32]),
+    "lbm3d_macros": (ctypes.c_int, [_vp, _pp, ctypes.c_int, _vp, _vp, _vp, _vp, _i64, _vp, _vp]),
+    "lbm3d_scatter_dense": (ctypes.c_int, [_vp, _pp, _vp, _vp, _vp, _i32, _i32, _i32, _vp, _vp]),
+    "lbm3d_reduce_workspace_bytes": (_sz, [_i32]),
+    "lbm3d_reduce": (ctypes.c_int, [_vp, _pp, _vp, _vp, _vp, _i32, _i32, _vp, _vp, _sz]),
+}
+
+
+class LbmError(RuntimeError):
+    pass
+
+
+class Lib:
+    """Loaded liblbm3d.so with checked calls."""
+
+    def __init__(self, path: str = LIB_PATH):
+        if not os.path.exists(path):
+            raise ImportError(
+                f"{path} is missing: build it with taichi_lbm3d_amd._lib.build() "
+                "(hipcc --offload-arch=gfx950); the solver has no CPU fallback")
+        self.path = path
+        self.cdll = ctypes.CDLL(path)
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(self.cdll, name)
+            fn.restype = res
+            fn.argtypes = args
+        v = self.cdll.lbm3d_abi_version()
+        if v != ABI_VERSION:
+            raise ImportError(f"liblbm3d.so ABI {v}, expected {ABI_VERSION}: rebuild it")
+
+    def last_error(self) -> str:
+        msg = self.cdll.lbm3d_last_error()
+        return msg.decode() if msg else ""
+
+    def call(self, name: str, *args):
+        rc = getattr(self.cdll, name)(*args)
+        if rc != 0:
+            raise LbmError(f"{name} failed ({rc}): {self.last_error()}")
+        return rc
+
+    def size(self, name: str, *args) -> int:
+        return int(getattr(self.cdll, name)(*args))
+
+
+_LIB: Lib | None = None
+
+
+def lib() -> Lib:
+    global _LIB
+    if _LIB is None:
+        _LIB = Lib()
+    return _LIB
