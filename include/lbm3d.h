@@ -80,18 +80,20 @@ const char *lbm3d_last_error(void);
  * grid of n_dense = nx*ny*nz cells. */
 size_t lbm3d_compact_workspace_bytes(int64_t n_dense);
 
-/* Pass 1: classify every cell of `solid` (uint8 C-order (nx,ny,nz), nonzero =
- * solid, SP:173-177) as solid / interior fluid / face-owned fluid (owner per
- * the bc_mode of `p`), and write counts[0] = interior cells, counts[1] =
- * face-owned cells (device int32[2]).  Fields of `p` used: nx, ny, nz, bc_mode. */
-int lbm3d_compact_count(void *stream, const lbm3d_params *p, const uint8_t *solid,
+/* Pass 1: classify every cell of `solid` (the int8 C-order (nx,ny,nz) field of
+ * SP:57; a cell is fluid iff its value is 0, and counts as solid for the
+ * pressure-face inward test iff its value is > 0, SP:278) as solid / interior
+ * fluid / face-owned fluid (owner per the bc_mode of `p`), and write
+ * counts[0] = interior cells, counts[1] = face-owned cells (device int32[2]).
+ * Fields of `p` used: nx, ny, nz, bc_mode. */
+int lbm3d_compact_count(void *stream, const lbm3d_params *p, const int8_t *solid,
                         int32_t *counts, void *workspace, size_t workspace_bytes);
 
 /* Pass 2 (after pass 1 with the same workspace): number the fluid cells.
  * Writes dense_to_compact[n_dense] (compact id or -1 for solid), cells[n_cells]
  * (C-order dense index of each fluid cell), face_tag[n_cells - n_interior].
  * p->n_interior and p->n_cells must hold the counts of pass 1. */
-int lbm3d_compact_fill(void *stream, const lbm3d_params *p, const uint8_t *solid,
+int lbm3d_compact_fill(void *stream, const lbm3d_params *p, const int8_t *solid,
                        int32_t *dense_to_compact, int32_t *cells, int32_t *face_tag,
                        void *workspace, size_t workspace_bytes);
 

@@ -120,7 +120,7 @@ class COracle:
         self._api["set_threads"](self._h, int(n))
 
     def set_solid(self, solid):
-        s = np.ascontiguousarray((np.asarray(solid) > 0).astype(np.int8))
+        s = np.ascontiguousarray(np.asarray(solid).astype(np.int8))
         if s.shape != (self.nx, self.ny, self.nz):
             raise ValueError("solid shape mismatch")
         self._solid = s

@@ -21,7 +21,9 @@ from .oracle_np import E
 
 def face_owner(solid: np.ndarray, bc_mode) -> tuple[np.ndarray, np.ndarray]:
     """owner face per cell (-1 none) and the face tag (owner | mode<<3 | inward_solid<<5)."""
-    solid = np.asarray(solid) != 0
+    raw = np.asarray(solid)
+    positive = raw > 0          # "solid" for the inward test, SP:278
+    solid = raw != 0            # not fluid, SP:225
     nx, ny, nz = solid.shape
     n = (nx, ny, nz)
     owner = np.full(solid.shape, -1, np.int32)
@@ -38,7 +40,7 @@ def face_owner(solid: np.ndarray, bc_mode) -> tuple[np.ndarray, np.ndarray]:
         sq = [slice(None)] * 3
         sq[axis] = inw
         owner[tuple(sl)] = f
-        tag[tuple(sl)] = f | (m << 3) | (solid[tuple(sq)].astype(np.int32) << 5)
+        tag[tuple(sl)] = f | (m << 3) | (positive[tuple(sq)].astype(np.int32) << 5)
     owner[solid] = -1
     tag[solid] = 0
     return owner, tag
@@ -46,8 +48,8 @@ def face_owner(solid: np.ndarray, bc_mode) -> tuple[np.ndarray, np.ndarray]:
 
 def compact(solid: np.ndarray, bc_mode=(0,) * 6):
     """(cells, n_interior, face_tag, dense_to_compact) with numpy."""
-    solid = np.asarray(solid) != 0
     owner, tag = face_owner(solid, bc_mode)
+    solid = np.asarray(solid) != 0
     fluid = ~solid
     interior = np.flatnonzero((fluid & (owner < 0)).reshape(-1))
     face = np.flatnonzero((fluid & (owner >= 0)).reshape(-1))

@@ -413,9 +413,9 @@ constexpr int kItems = 4;
 constexpr int kTile = kBlock * kItems;
 
 // class of dense cell c: 0 solid, 1 interior fluid, 2 face-owned fluid (tag set)
-__device__ __forceinline__ int classify(const lbm3d_params &p, const uint8_t *__restrict__ solid,
+__device__ __forceinline__ int classify(const lbm3d_params &p, const int8_t *__restrict__ solid,
                                         int64_t c, int32_t &tag) {
-  if (solid[c]) return 0;
+  if (solid[c] != 0) return 0;  // fluid iff solid == 0 (SP:225)
   const int nyz = p.ny * p.nz;
   const int i = (int)(c / nyz);
   const int j = (int)((c / p.nz) % p.ny);
@@ -436,13 +436,13 @@ __device__ __forceinline__ int classify(const lbm3d_params &p, const uint8_t *__
   const int n = a == 0 ? p.nx : (a == 1 ? p.ny : p.nz);
   const int inw = (owner & 1) ? n - 2 : 1;
   if (a == 0) ii = inw; else if (a == 1) jj = inw; else kk = inw;
-  const int inward_solid = solid[((int64_t)ii * p.ny + jj) * p.nz + kk] ? 1 : 0;
+  const int inward_solid = solid[((int64_t)ii * p.ny + jj) * p.nz + kk] > 0 ? 1 : 0;  // SP:278
   tag = owner | (p.bc_mode[owner] << 3) | (inward_solid << 5);
   return 2;
 }
 
 __global__ void __launch_bounds__(kBlock) k_compact_count(const lbm3d_params p,
-                                                          const uint8_t *__restrict__ solid,
+                                                          const int8_t *__restrict__ solid,
                                                           int64_t n_dense, int32_t *__restrict__ tile_cnt) {
   __shared__ int s_cnt[2];
   if (threadIdx.x == 0) s_cnt[0] = s_cnt[1] = 0;
@@ -514,7 +514,7 @@ __global__ void __launch_bounds__(kBlock) k_compact_scan(int32_t *__restrict__ t
 }
 
 __global__ void __launch_bounds__(kBlock) k_compact_fill(const lbm3d_params p,
-                                                         const uint8_t *__restrict__ solid,
+                                                         const int8_t *__restrict__ solid,
                                                          int64_t n_dense,
                                                          const int32_t *__restrict__ tile_off,
                                                          int32_t *__restrict__ d2c,
@@ -737,7 +737,7 @@ size_t lbm3d_compact_workspace_bytes(int64_t n_dense) {
   return (size_t)(tiles > 0 ? tiles : 1) * 2 * sizeof(int32_t);
 }
 
-int lbm3d_compact_count(void *stream, const lbm3d_params *p, const uint8_t *solid, int32_t *counts,
+int lbm3d_compact_count(void *stream, const lbm3d_params *p, const int8_t *solid, int32_t *counts,
                         void *workspace, size_t workspace_bytes) {
   int rc = check_params(p);
   if (rc) return rc;
@@ -753,7 +753,7 @@ int lbm3d_compact_count(void *stream, const lbm3d_params *p, const uint8_t *soli
   return check_launch("lbm3d_compact_count/scan");
 }
 
-int lbm3d_compact_fill(void *stream, const lbm3d_params *p, const uint8_t *solid, int32_t *d2c,
+int lbm3d_compact_fill(void *stream, const lbm3d_params *p, const int8_t *solid, int32_t *d2c,
                        int32_t *cells, int32_t *face_tag, void *workspace, size_t workspace_bytes) {
   int rc = check_params(p);
   if (rc) return rc;

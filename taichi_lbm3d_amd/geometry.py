@@ -12,14 +12,16 @@ def read_geo(path: str, nx: int, ny: int, nz: int) -> np.ndarray:
     """ASCII voxel file -> solid mask, as init_geo does (SP:173-177).
 
     One integer per line, x fastest, then y, then z (README.md:17-23); any
-    value > 0 is solid.  Loaded with ``np.loadtxt`` then reshaped in Fortran
-    order to ``(nx, ny, nz)``.
+    value > 0 becomes 1.  Loaded with ``np.loadtxt`` then reshaped in Fortran
+    order to ``(nx, ny, nz)`` and cast to int8 like the i8 ``solid`` field, so
+    a negative value stays negative: such a cell is not fluid (the kernels test
+    ``solid == 0``) yet not solid for the pressure-face test ``solid > 0``.
     """
     data = np.loadtxt(path)
     data[data > 0] = 1
     if data.size != nx * ny * nz:
         raise ValueError(f"{path}: {data.size} values, expected {nx}*{ny}*{nz}")
-    return np.ascontiguousarray(np.reshape(data, (nx, ny, nz), order="F")).astype(np.uint8)
+    return np.ascontiguousarray(np.reshape(data, (nx, ny, nz), order="F")).astype(np.int8)
 
 
 def write_geo(path: str, solid: np.ndarray) -> None:

@@ -282,7 +282,7 @@ class LB3D_Solver_Single_Phase:
         dev = self._torch_device()
         p = self._params
         n_dense = self.nx * self.ny * self.nz
-        solid = torch.from_numpy((self.solid._data != 0).astype(np.uint8).reshape(-1)).to(dev)
+        solid = torch.from_numpy(np.ascontiguousarray(self.solid._data.reshape(-1))).to(dev)
         ws_bytes = lib.size("lbm3d_compact_workspace_bytes", n_dense)
         ws = torch.empty(ws_bytes, dtype=torch.uint8, device=dev)
         counts = torch.zeros(2, dtype=torch.int32, device=dev)

@@ -1,0 +1,234 @@
+"""CPU tests: fixtures, compaction restatements, the C ABI surface and host logic."""
+import ctypes
+import hashlib
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+from oracle import compaction as oc
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+GOLDEN = os.path.join(ROOT, "tests", "golden")
+
+
+def load_mask(name, shape):
+    d = np.load(os.path.join(GOLDEN, name))
+    bits = np.unpackbits(d["bits"])[: int(d["n"])]
+    return bits.reshape(shape, order="F"), int(d["n_solid"])
+
+
+# ---- fixtures ---------------------------------------------------------------
+
+def test_cavity_fixture_matches_generator():
+    """geo_cavity.dat (50^3) is the mask flow_domain_geo_generation_2D.py:15-27 writes."""
+    from taichi_lbm3d_amd.geometry import cavity
+    solid, n_solid = load_mask("geo_cavity.npz", (50, 50, 50))
+    assert n_solid == 12104 and int((solid == 0).sum()) == 112896
+    np.testing.assert_array_equal(solid, cavity(50))
+
+
+def test_read_geo_layout(tmp_path):
+    """init_geo reads one value per line, x fastest (SP:173-177, README.md:17-23)."""
+    from taichi_lbm3d_amd.geometry import read_geo, write_geo
+    rng = np.random.default_rng(0)
+    a = (rng.random((5, 4, 3)) < 0.4).astype(np.uint8)
+    p = tmp_path / "g.dat"
+    write_geo(str(p), a)
+    lines = p.read_text().split()
+    assert lines[1] == str(a[1, 0, 0]) and lines[5] == str(a[0, 1, 0]) and lines[20] == str(a[0, 0, 1])
+    np.testing.assert_array_equal(read_geo(str(p), 5, 4, 3), a)
+    p.write_text("\n".join(["0", "2", "0.5", "-1"] * 15))  # values > 0 are solid
+    g = read_geo(str(p), 5, 4, 3)
+    assert g.dtype == np.int8 and g.reshape(-1, order="F")[:4].tolist() == [0, 1, 1, -1]
+
+
+# ---- compaction -------------------------------------------------------------
+
+@pytest.mark.parametrize("seed", range(6))
+def test_compaction_loops_equal_numpy(seed):
+    rng = np.random.default_rng(seed)
+    shape = tuple(int(x) for x in rng.integers(2, 7, 3))
+    # values of the i8 field: 0 fluid, > 0 solid, < 0 neither fluid nor "solid > 0"
+    solid = rng.choice(np.array([0, 1, -1, 2], np.int8), size=shape, p=[0.7, 0.15, 0.1, 0.05])
+    modes = tuple(int(x) for x in rng.integers(0, 3, 6))
+    cells, n_int, tags, d2c = oc.compact(solid, modes)
+    nbr = oc.neighbors(solid, cells, d2c)
+    c2, n2, t2, nb2 = oc.compact_loops(solid, modes)
+    np.testing.assert_array_equal(cells, c2)
+    assert n_int == n2
+    np.testing.assert_array_equal(tags, t2)
+    np.testing.assert_array_equal(nbr, nb2)
+
+
+def test_compaction_cavity_checksum():
+    solid, _ = load_mask("geo_cavity.npz", (50, 50, 50))
+    g = np.load(os.path.join(GOLDEN, "cavity_compaction.npz"))
+    cells, n_int, tags, d2c = oc.compact(solid, tuple(g["modes"]))
+    nbr = oc.neighbors(solid, cells, d2c)
+    assert cells.size == int(g["n_cells"]) == 112896 and n_int == int(g["n_interior"])
+    assert hashlib.sha256(cells.astype("<i4").tobytes()).hexdigest() == str(g["sha_cells"])
+    assert hashlib.sha256(nbr.astype("<i4").tobytes()).hexdigest() == str(g["sha_nbr"])
+    np.testing.assert_array_equal(tags, g["face_tag"])
+
+
+def test_compaction_all_fluid_docs_fixture():
+    """docs/geo.dat: 48^3 all fluid; every neighbour exists (periodic wrap, no bounce-back)."""
+    solid, n_solid = load_mask("geo_docs48.npz", (48, 48, 48))
+    assert n_solid == 0
+    cells, n_int, _, d2c = oc.compact(solid)
+    nbr = oc.neighbors(solid, cells, d2c)
+    assert cells.size == n_int == 48 ** 3 and nbr.min() >= 0
+    # each direction is a permutation of the cells
+    for q in range(18):
+        assert np.unique(nbr[q]).size == cells.size
+
+
+# ---- the C ABI --------------------------------------------------------------
+
+def _header_functions():
+    txt = open(os.path.join(ROOT, "include", "lbm3d.h")).read()
+    txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+    return sorted(set(re.findall(r"\b(lbm3d_[a-z_]+)\s*\(", txt)) - {"lbm3d_params"})
+
+
+def test_abi_exports_every_declared_symbol():
+    from taichi_lbm3d_amd import _lib
+    lib = _lib.lib()
+    names = _header_functions()
+    assert len(names) >= 12
+    for n in names:
+        assert hasattr(lib.cdll, n), n
+        assert n in _lib.SIGNATURES, n
+    assert lib.cdll.lbm3d_abi_version() == _lib.ABI_VERSION
+    out = subprocess.run(["nm", "-D", "--defined-only", lib.path], capture_output=True, text=True).stdout
+    exported = set(re.findall(r"\b(lbm3d_\w+)\b", out))
+    assert set(names) <= exported
+
+
+def test_params_struct_layout(tmp_path):
+    """ctypes mirror of lbm3d_params has the C layout (size and every offset)."""
+    from taichi_lbm3d_amd._lib import Params
+    fields = [f[0] for f in Params._fields_]
+    src = tmp_path / "layout.c"
+    body = "".join(f'printf("{f} %zu\\n", offsetof(lbm3d_params, {f}));' for f in fields)
+    src.write_text('#include <stdio.h>\n#include <stddef.h>\n#include "lbm3d.h"\nint main(void){'
+                   f'printf("size %zu\\n", sizeof(lbm3d_params));{body}return 0;}}\n')
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), "-o", str(exe), str(src)], check=True)
+    got = dict(line.split() for line in subprocess.run([str(exe)], capture_output=True, text=True).stdout.split("\n")
+               if line)
+    assert int(got["size"]) == ctypes.sizeof(Params)
+    for f in fields:
+        assert int(got[f]) == getattr(Params, f).offset, f
+
+
+def test_abi_rejects_bad_arguments_without_gpu():
+    """Argument checks run before any HIP call and report through lbm3d_last_error."""
+    from taichi_lbm3d_amd import _lib
+    lib = _lib.lib()
+    p = _lib.Params()
+    p.nx = p.ny = p.nz = 4
+    p.n_cells, p.n_interior = 10, 20
+    rc = lib.cdll.lbm3d_step(None, ctypes.byref(p), 0, None, None, None, None, 0)
+    assert rc == -1 and "n_interior" in lib.last_error()
+    p.n_cells = p.n_interior = 10
+    rc = lib.cdll.lbm3d_step(None, ctypes.byref(p), 7, ctypes.c_void_p(8), ctypes.c_void_p(8), None, None, 0)
+    assert rc == -1 and "dtype" in lib.last_error()
+    p.n_cells = 300_000_000
+    p.n_interior = p.n_cells
+    rc = lib.cdll.lbm3d_step(None, ctypes.byref(p), 0, ctypes.c_void_p(8), ctypes.c_void_p(8), None, None, 0)
+    assert rc == -3
+    p.n_cells = p.n_interior = 10
+    p.nx = 1
+    p.bc_mode[0] = 1
+    rc = lib.cdll.lbm3d_compact_count(None, ctypes.byref(p), ctypes.c_void_p(8), ctypes.c_void_p(8),
+                                      ctypes.c_void_p(8), 1 << 20)
+    assert rc == -1 and "extent" in lib.last_error()
+    assert lib.cdll.lbm3d_compact_workspace_bytes(2 * 1024) == 16
+
+
+# ---- host logic ---------------------------------------------------------------
+
+def test_relaxation_rates():
+    from taichi_lbm3d_amd import relaxation_rates
+    tau, sv, so = relaxation_rates(0.1)
+    assert abs(tau - (0.1 / 3 + 0.5)) < 1e-15 and abs(sv - 1 / tau) < 1e-15
+    assert abs(so - 8 * (2 - sv) / (8 - sv)) < 1e-15
+    tau, _, _ = relaxation_rates(0.1, "documented")
+    assert abs(tau - 0.8) < 1e-15
+    with pytest.raises(ValueError):
+        relaxation_rates(0.1, "nope")
+
+
+def test_solver_surface_without_gpu():
+    """The reference's attribute surface and setters (SP:13-28, SP:405-458)."""
+    from taichi_lbm3d_amd import LB3D_Solver_Single_Phase
+    s = LB3D_Solver_Single_Phase(5, 4, 3)
+    assert (s.nx, s.ny, s.nz) == (5, 4, 3) and s.niu == 0.16667 and s.sparse_storage is False
+    assert s.bc_x_left == 0 and s.rho_bcxl == 1.0 and s.vx_bcxl == 0.0e-5
+    s.set_bc_vel_x1([0.0, 0.0, 0.1])
+    assert s.bc_x_right == 2 and (s.vx_bcxr, s.vy_bcxr, s.vz_bcxr) == (0.0, 0.0, 0.1)
+    s.set_bc_rho_y0(1.01)
+    assert s.bc_y_left == 1 and s.rho_bcyl == 1.01
+    s.set_force([1e-5, 0, 0])
+    s.set_viscosity(0.2)
+    assert s.fx == 1e-5 and s.niu == 0.2
+    s.solid.from_numpy(np.ones((5, 4, 3)) * 0.0)
+    assert s.solid.to_numpy().dtype == np.int8 and s.solid.shape == (5, 4, 3)
+    with pytest.raises(ValueError):
+        s.solid.from_numpy(np.zeros((4, 4, 4)))
+    assert s.rho.to_numpy().shape == (5, 4, 3) and s.v.to_numpy().shape == (5, 4, 3, 3)
+    sp = LB3D_Solver_Single_Phase(5, 4, 3, sparse_storage=True)
+    assert sp.rho.shape == (6, 6, 6)
+    import torch
+    if not torch.cuda.is_available():
+        with pytest.raises(RuntimeError):
+            s.init_simulation()
+        with pytest.raises(RuntimeError):
+            s.step()
+
+
+# ---- geometry generators and VTK -----------------------------------------------
+
+def test_sphere_pack_properties():
+    from taichi_lbm3d_amd.geometry import sphere_pack
+    a = sphere_pack((48, 40, 40), porosity=0.4, rmin=3, rmax=6, seed=1)
+    b = sphere_pack((48, 40, 40), porosity=0.4, rmin=3, rmax=6, seed=1)
+    np.testing.assert_array_equal(a, b)
+    phi = (a == 0).mean()
+    assert 0.38 < phi <= 0.4
+    c = sphere_pack((48, 40, 40), porosity=0.4, rmin=3, rmax=6, seed=2)
+    assert not np.array_equal(a, c)
+    # periodic in y and z: solid touching y = 0 continues at y = ny-1 somewhere
+    assert (a[:, 0, :] & a[:, -1, :]).any() and (a[:, :, 0] & a[:, :, -1]).any()
+
+
+def test_cube_channel_properties():
+    from taichi_lbm3d_amd.geometry import cube_channel
+    g = cube_channel((40, 32, 32), fraction=0.05, seed=0)
+    assert g[:, 0].all() and g[:, -1].all() and g[:, :, 0].all() and g[:, :, -1].all()
+    inner = g[:, 1:-1, 1:-1]
+    assert 0.04 < inner.mean() < 0.12
+
+
+def test_vtk_roundtrip(tmp_path):
+    from taichi_lbm3d_amd.vtk import grid_to_vtk, read_vtr
+    rng = np.random.default_rng(0)
+    n = (5, 4, 3)
+    solid = (rng.random(n) < 0.5).astype(np.int8)
+    rho = rng.random(n).astype(np.float32)
+    v = [rng.random(n).astype(np.float32) for _ in range(3)]
+    x, y, z = (np.linspace(0, k, k) for k in n)
+    fn = grid_to_vtk(str(tmp_path / "LB_SingelPhase_7"), x, y, z,
+                     pointData={"Solid": solid, "rho": rho, "velocity": tuple(v)})
+    assert fn.endswith("LB_SingelPhase_7.vtr")
+    back = read_vtr(fn)
+    np.testing.assert_array_equal(back["Solid"], solid)
+    np.testing.assert_array_equal(back["rho"], rho)
+    np.testing.assert_array_equal(back["velocity"], np.stack(v, -1))
+    np.testing.assert_array_equal(back["x"], x)
+    head = open(fn, "rb").read(400).decode("ascii", "replace")
+    assert 'type="RectilinearGrid"' in head and 'WholeExtent="0 4 0 3 0 2"' in head
