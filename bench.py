@@ -112,7 +112,7 @@ def main():
     solid = make_geometry(args.n, args.seed)
     s = make_solver(solid, args.dtype)
     n_fluid = s.n_fluid
-    log(f"fluid cells {n_fluid}, interior {s._params.n_interior}")
+    log(f"fluid cells {n_fluid}")
 
     s.run(args.warmup)
     torch.cuda.synchronize()

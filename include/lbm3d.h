@@ -15,12 +15,17 @@
  *     enqueued; nothing synchronises unless stated.
  *   - Return 0 on success, a negative LBM3D_E* code otherwise;
  *     lbm3d_last_error() then gives a message (per thread).
+ *   - Fluid cells (solid value 0) are numbered 0 .. n_cells-1 in C order of
+ *     (i, j, k), k fastest.
  *   - Populations are stored shifted, g_q = f_q - w_q, in SoA order
- *     g[q * n_cells + cell], q = 0..18 in the reference's direction order
- *     (SP:183-187), in the storage type `dtype` (LBM3D_F32/F16/BF16);
- *     arithmetic is FP32.
- *   - Fluid cells are numbered: interior cells first, then cells owned by a
- *     pressure/velocity face, each group in C order (i, j, k; k fastest).
+ *     g[q * S + cell], q = 0..18 in the reference's direction order
+ *     (SP:183-187), S = lbm3d_slot_stride(n_cells) (n_cells rounded up to a
+ *     multiple of 64), in the storage type `dtype` (LBM3D_F32/F16/BF16), so
+ *     g holds 19 * S elements; arithmetic is FP32.
+ *   - Between steps g is in the AA pattern's layout for the step count t:
+ *     even t, population q of a cell in its own slot q; odd t, in slot
+ *     opp(q) of the upstream neighbour (or in the cell's own slot q where
+ *     that neighbour is not fluid).
  */
 #ifndef LBM3D_H
 #define LBM3D_H
@@ -32,7 +37,7 @@
 extern "C" {
 #endif
 
-#define LBM3D_ABI_VERSION 1
+#define LBM3D_ABI_VERSION 4
 
 /* storage types for populations */
 #define LBM3D_F32 0
@@ -52,26 +57,47 @@ extern "C" {
 /* Frozen solver parameters: what SP bakes into its kernels at
  * init_simulation() and first compilation (SP:118-149, SP:230, SP:274-365). */
 typedef struct lbm3d_params {
-  int32_t nx, ny, nz;  /* grid extent (SP:11) */
-  int32_t n_cells;     /* fluid cells after compaction */
-  int32_t n_interior;  /* cells [0, n_interior) are interior, the rest face-owned */
-  int32_t force_on;    /* force_flag (SP:137-140) */
-  float s_v, s_o;      /* relaxation rates s_v and s_other (SP:127-129) */
-  float fx, fy, fz;    /* body force (SP:17, SP:134-136) */
-  float guo_a, guo_b;  /* force-term scalings: SP uses 1/3 and 1/9 (SP:236) */
-  int32_t bc_mode[6];  /* 0 periodic, 1 fixed pressure, 2 fixed velocity (SP:22) */
-  float rho_bc[6];     /* rho_bcxl ... (SP:23-28) */
-  float u_bc[6][3];    /* vx_bcxl, vy_bcxl, vz_bcxl ... (SP:23-28) */
+  int32_t nx, ny, nz;      /* grid extent (SP:11) */
+  int32_t n_cells;         /* fluid cells after compaction */
+  int32_t force_on;        /* force_flag (SP:137-140) */
+  float s_v, s_o;          /* relaxation rates s_v and s_other (SP:127-129) */
+  float fx, fy, fz;        /* body force (SP:17, SP:134-136) */
+  float guo_a, guo_b;      /* force-term scalings: SP uses 1/3 and 1/9 (SP:236) */
+  int32_t bc_mode[6];      /* 0 periodic, 1 fixed pressure, 2 fixed velocity (SP:22) */
+  float rho_bc[6];         /* rho_bcxl ... (SP:23-28) */
+  float u_bc[6][3];        /* vx_bcxl, vy_bcxl, vz_bcxl ... (SP:23-28) */
+  int32_t face_u_off[6];   /* float offset of each pressure face's state in face_u, -1 if unused */
 } lbm3d_params;
 
-/* Face tag of a face-owned cell: bits 0-2 owner face, bits 3-4 mode, bit 5
- * set when the owner's inward neighbour is solid (SP:278, :294, ...). */
-#define LBM3D_TAG_FACE(t) ((t) & 7)
-#define LBM3D_TAG_MODE(t) (((t) >> 3) & 3)
-#define LBM3D_TAG_INWARD_SOLID(t) (((t) >> 5) & 1)
+/* One 32-cell word of a grid row (i, j): bit b of `mask` set when cell
+ * (i, j, 32w + b) is fluid; `base` = compact id of the first fluid cell of
+ * the word; `left` / `right` = compact id of the cell just before 32w and
+ * just after 32w+31 along k (with the periodic wrap of SP:247-257 at k = 0
+ * and k = nz-1), or -1 when that cell is not fluid.
+ * rows[(i * ny + j) * W + w], W = (nz + 31) / 32. */
+typedef struct lbm3d_row_word {
+  uint32_t mask;
+  int32_t base;
+  int32_t left;
+  int32_t right;
+} lbm3d_row_word;
 
 int lbm3d_abi_version(void);
 const char *lbm3d_last_error(void);
+
+/* ---- sizes ---------------------------------------------------------------- */
+
+/* Elements per population slot for n_cells fluid cells (a multiple of 64). */
+int64_t lbm3d_slot_stride(int32_t n_cells);
+/* Entries of the row table: nx * ny * ((nz + 31) / 32). */
+int64_t lbm3d_row_words(const lbm3d_params *p);
+/* uint32 words of face_bits: 2 bits per fluid cell, (n_cells + 15) / 16. */
+int64_t lbm3d_face_bit_words(int32_t n_cells);
+/* Nodes in the plane of face f (ny*nz, nx*nz or nx*ny).  The state of a
+ * pressure face f is 3 * lbm3d_face_plane(p, f) floats at face_u +
+ * face_u_off[f]: component c of the node with in-plane index
+ * (x faces j*nz + k, y faces i*nz + k, z faces i*ny + j) at c * plane + index. */
+int64_t lbm3d_face_plane(const lbm3d_params *p, int32_t face);
 
 /* ---- geometry: the compacted fluid list (replaces SP's pointer/dense SNode
  * tree, SP:41-44, and the fluid test at SP:225, :262, :376) ---------------- */
@@ -80,55 +106,64 @@ const char *lbm3d_last_error(void);
  * grid of n_dense = nx*ny*nz cells. */
 size_t lbm3d_compact_workspace_bytes(int64_t n_dense);
 
-/* Pass 1: classify every cell of `solid` (the int8 C-order (nx,ny,nz) field of
- * SP:57; a cell is fluid iff its value is 0, and counts as solid for the
- * pressure-face inward test iff its value is > 0, SP:278) as solid / interior
- * fluid / face-owned fluid (owner per the bc_mode of `p`), and write
- * counts[0] = interior cells, counts[1] = face-owned cells (device int32[2]).
- * Fields of `p` used: nx, ny, nz, bc_mode. */
-int lbm3d_compact_count(void *stream, const lbm3d_params *p, const int8_t *solid,
-                        int32_t *counts, void *workspace, size_t workspace_bytes);
+/* Pass 1: count the fluid cells of `solid` (the int8 C-order (nx,ny,nz) field
+ * of SP:57; a cell is fluid iff its value is 0) into count[0] (device int32). */
+int lbm3d_compact_count(void *stream, const lbm3d_params *p, const int8_t *solid, int32_t *count,
+                        void *workspace, size_t workspace_bytes);
 
-/* Pass 2 (after pass 1 with the same workspace): number the fluid cells.
- * Writes dense_to_compact[n_dense] (compact id or -1 for solid), cells[n_cells]
- * (C-order dense index of each fluid cell), face_tag[n_cells - n_interior].
- * p->n_interior and p->n_cells must hold the counts of pass 1. */
-int lbm3d_compact_fill(void *stream, const lbm3d_params *p, const int8_t *solid,
-                       int32_t *dense_to_compact, int32_t *cells, int32_t *face_tag,
-                       void *workspace, size_t workspace_bytes);
+/* Pass 2 (after pass 1 with the same workspace; p->n_cells = the count):
+ * cells[n_cells] = C-order dense index of each fluid cell, and, when d2c is
+ * not NULL, d2c[nx*ny*nz] = compact id or -1. */
+int lbm3d_compact_fill(void *stream, const lbm3d_params *p, const int8_t *solid, int32_t *d2c,
+                       int32_t *cells, void *workspace, size_t workspace_bytes);
 
-/* Neighbour table, SoA nbr[(q-1) * n_cells + cell] for q = 1..18: the compact
- * id of the cell at (i,j,k)+e_q with the unconditional periodic wrap of SP's
- * periodic_index (SP:247-257), or -1 when that cell is solid, i.e. when
- * streaming1 bounces f_q back into the cell's own opposite slot (SP:265-268). */
-int lbm3d_build_neighbors(void *stream, const lbm3d_params *p, const int32_t *dense_to_compact,
-                          const int32_t *cells, int32_t *nbr);
+/* Row table (see lbm3d_row_word) from solid and the d2c map of pass 2. */
+int lbm3d_build_rows(void *stream, const lbm3d_params *p, const int8_t *solid, const int32_t *d2c,
+                     lbm3d_row_word *rows);
+
+/* face_bits: per fluid cell, bit 2*(x%16) of word x/16 set when the cell
+ * belongs to a face in mode 1 or 2 (owner = last such face in face order),
+ * bit 2*(x%16)+1 set when the owner's inward neighbour (index 1 or n-2 along
+ * the face normal) has solid value > 0 (SP:278, :294, ...). */
+int lbm3d_build_face_bits(void *stream, const lbm3d_params *p, const int8_t *solid,
+                          const int32_t *cells, uint32_t *face_bits);
+
+/* Explicit neighbour table, SoA nbr[(q-1) * n_cells + cell] for q = 1..18:
+ * the compact id of the cell at (i,j,k)+e_q under the unconditional periodic
+ * wrap of SP's periodic_index (SP:247-257), or -1 when that cell is not
+ * fluid, i.e. when streaming1 bounces f_q back into the cell's own opposite
+ * slot (SP:265-268).  The step kernel computes the same ids from `rows` on
+ * the fly, with the same device function; this table is for inspection and
+ * tests. */
+int lbm3d_build_neighbors(void *stream, const lbm3d_params *p, const int32_t *cells,
+                          const lbm3d_row_word *rows, int32_t *nbr);
 
 /* ---- solver ------------------------------------------------------------- */
 
-/* init (SP:160-170): g = feq(rho0, u0) - w on every fluid cell, face_u = u0 on
- * face-owned cells.  rho0 [n_cells] and u0 (SoA [3][n_cells]) may be NULL for
- * the reference's rest state rho = 1, u = 0.  face_u is SoA [3][n_face]. */
-int lbm3d_init(void *stream, const lbm3d_params *p, int dtype, void *g, float *face_u,
-               const float *rho0, const float *u0);
+/* init (SP:160-170): g = feq(rho0, u0) - w on every fluid cell, and the
+ * pressure-face state = u0 at the face nodes.  rho0 [n_cells] and u0 (SoA
+ * [3][n_cells]) may be NULL for the reference's rest state rho = 1, u = 0. */
+int lbm3d_init(void *stream, const lbm3d_params *p, int dtype, void *g, const int32_t *cells,
+               float *face_u, const float *rho0, const float *u0);
 
 /* One time step: the reference's colission -> streaming1 ->
  * Boundary_condition -> streaming3 (SP:477-481) as one fused kernel over the
- * fluid list, in-place with the AA access pattern.  `t` is the number of steps
- * already taken; its parity selects the access pattern, and t == 0 is the
- * step that starts from init's state (macros rho0, u0 without the half-force
- * shift, face populations as initialised). */
-int lbm3d_step(void *stream, const lbm3d_params *p, int dtype, void *g, const int32_t *nbr,
-               const int32_t *face_tag, float *face_u, int64_t t);
+ * fluid list, in place with the AA access pattern.  `t` is the number of
+ * steps already taken; its parity selects the access pattern, and t == 0 is
+ * the step that starts from init's state (macros rho0, u0 without the
+ * half-force shift, face populations as initialised). */
+int lbm3d_step(void *stream, const lbm3d_params *p, int dtype, void *g, const int32_t *cells,
+               const lbm3d_row_word *rows, const uint32_t *face_bits, float *face_u, int64_t t);
 
 /* `nsteps` consecutive lbm3d_step calls from step t0, enqueued back to back. */
-int lbm3d_run(void *stream, const lbm3d_params *p, int dtype, void *g, const int32_t *nbr,
-              const int32_t *face_tag, float *face_u, int64_t t0, int32_t nsteps);
+int lbm3d_run(void *stream, const lbm3d_params *p, int dtype, void *g, const int32_t *cells,
+              const lbm3d_row_word *rows, const uint32_t *face_bits, float *face_u, int64_t t0,
+              int32_t nsteps);
 
 /* Macroscopic fields after t steps, what SP's streaming3 leaves in rho and v
  * (SP:376-388): rho[n_cells] and v (SoA [3][n_cells]), FP32. */
-int lbm3d_macros(void *stream, const lbm3d_params *p, int dtype, const void *g,
-                 const int32_t *nbr, const int32_t *face_tag, const float *face_u, int64_t t,
+int lbm3d_macros(void *stream, const lbm3d_params *p, int dtype, const void *g, const int32_t *cells,
+                 const lbm3d_row_word *rows, const uint32_t *face_bits, const float *face_u, int64_t t,
                  float *rho, float *v);
 
 /* Scatter compact rho/v into dense C-order arrays of extent (ox,oy,oz) >=

@@ -7,10 +7,11 @@ pushing ``F[ip][s] = f[i][s]`` when ``ip`` is fluid and bouncing back into
 information as a numbered fluid list and a table; these functions restate it
 two ways (plain loops, and numpy) so the GPU build can be checked bit for bit.
 
-Numbering: interior fluid cells first, then cells owned by a face with a
-pressure or velocity condition, each group in C order.  A node on several such
-faces belongs to the last of x-left, x-right, y-left, y-right, z-left, z-right,
-the order Boundary_condition writes them (SP:274-370).
+Numbering: fluid cells in C order.  Face records: a fluid node on a face with
+a pressure or velocity condition belongs to the last such face of x-left,
+x-right, y-left, y-right, z-left, z-right, the order Boundary_condition
+writes them (SP:274-370); its record is 1, plus 2 when that face's inward
+neighbour has solid value > 0 (SP:278).
 """
 from __future__ import annotations
 
@@ -46,17 +47,31 @@ def face_owner(solid: np.ndarray, bc_mode) -> tuple[np.ndarray, np.ndarray]:
     return owner, tag
 
 
-def compact(solid: np.ndarray, bc_mode=(0,) * 6):
-    """(cells, n_interior, face_tag, dense_to_compact) with numpy."""
-    owner, tag = face_owner(solid, bc_mode)
-    solid = np.asarray(solid) != 0
-    fluid = ~solid
-    interior = np.flatnonzero((fluid & (owner < 0)).reshape(-1))
-    face = np.flatnonzero((fluid & (owner >= 0)).reshape(-1))
-    cells = np.concatenate([interior, face]).astype(np.int32)
-    d2c = np.full(solid.size, -1, np.int32)
+def compact(solid: np.ndarray):
+    """(cells, dense_to_compact) with numpy: fluid cells (value 0) in C order."""
+    fluid = (np.asarray(solid) == 0).reshape(-1)
+    cells = np.flatnonzero(fluid).astype(np.int32)
+    d2c = np.full(fluid.size, -1, np.int32)
     d2c[cells] = np.arange(cells.size, dtype=np.int32)
-    return cells, int(interior.size), tag.reshape(-1)[face].astype(np.int32), d2c
+    return cells, d2c
+
+
+def face_records(solid: np.ndarray, cells: np.ndarray, bc_mode) -> np.ndarray:
+    """Per fluid cell: 0, or 1 for a face-owned node, +2 if the owner's inward neighbour is solid."""
+    owner, tag = face_owner(solid, bc_mode)
+    own = owner.reshape(-1)[cells]
+    inward = (tag.reshape(-1)[cells] >> 5) & 1
+    return np.where(own >= 0, 1 + 2 * inward, 0).astype(np.uint32)
+
+
+def pack_face_bits(rec: np.ndarray) -> np.ndarray:
+    """2-bit records packed 16 per uint32 word, cell x at bits 2*(x%16)."""
+    n = rec.size
+    words = np.zeros((n + 15) // 16, np.uint32)
+    for b in range(16):
+        part = rec[b::16].astype(np.uint32)
+        words[: part.size] |= part << np.uint32(2 * b)
+    return words
 
 
 def neighbors(solid: np.ndarray, cells: np.ndarray, d2c: np.ndarray) -> np.ndarray:
@@ -74,38 +89,36 @@ def neighbors(solid: np.ndarray, cells: np.ndarray, d2c: np.ndarray) -> np.ndarr
 
 
 def compact_loops(solid, bc_mode=(0,) * 6):
-    """The same numbering and table with plain Python loops (small grids only),
-    following periodic_index (SP:247-257) and streaming1 (SP:259-269) literally."""
+    """The same numbering, face records and table with plain Python loops
+    (small grids only), following periodic_index (SP:247-257), streaming1
+    (SP:259-269) and the face order of Boundary_condition (SP:274-370) literally."""
     solid = np.asarray(solid)
     nx, ny, nz = solid.shape
     order = []
-    for face_pass in (False, True):
-        for i in range(nx):
-            for j in range(ny):
-                for k in range(nz):
-                    if solid[i, j, k] != 0:
-                        continue
-                    owner = -1
-                    for f in range(6):
-                        if int(bc_mode[f]) not in (1, 2):
-                            continue
-                        axis, right = divmod(f, 2)
-                        pos = (i, j, k)[axis]
-                        if pos == ((nx, ny, nz)[axis] - 1 if right else 0):
-                            owner = f
-                    if (owner >= 0) == face_pass:
-                        order.append((i, j, k, owner))
-    ids = {(i, j, k): x for x, (i, j, k, _) in enumerate(order)}
-    n_int = sum(1 for o in order if o[3] < 0)
-    tags = []
-    for (i, j, k, owner) in order[n_int:]:
+    for i in range(nx):
+        for j in range(ny):
+            for k in range(nz):
+                if solid[i, j, k] == 0:
+                    order.append((i, j, k))
+    ids = {c: x for x, c in enumerate(order)}
+    rec = []
+    for (i, j, k) in order:
+        owner = -1
+        for f in range(6):
+            if int(bc_mode[f]) not in (1, 2):
+                continue
+            axis, right = divmod(f, 2)
+            if (i, j, k)[axis] == ((nx, ny, nz)[axis] - 1 if right else 0):
+                owner = f
+        if owner < 0:
+            rec.append(0)
+            continue
         axis, right = divmod(owner, 2)
-        n = (nx, ny, nz)[axis]
         q = [i, j, k]
-        q[axis] = n - 2 if right else 1
-        tags.append(owner | (int(bc_mode[owner]) << 3) | ((1 if solid[q[0], q[1], q[2]] > 0 else 0) << 5))
+        q[axis] = (nx, ny, nz)[axis] - 2 if right else 1
+        rec.append(1 + 2 * (1 if solid[q[0], q[1], q[2]] > 0 else 0))
     nbr = np.empty((18, len(order)), np.int32)
-    for x, (i, j, k, _) in enumerate(order):
+    for x, (i, j, k) in enumerate(order):
         for q in range(1, 19):
             ip = [i + int(E[q, 0]), j + int(E[q, 1]), k + int(E[q, 2])]
             if ip[0] < 0: ip[0] = nx - 1
@@ -114,6 +127,6 @@ def compact_loops(solid, bc_mode=(0,) * 6):
             if ip[1] > ny - 1: ip[1] = 0
             if ip[2] < 0: ip[2] = nz - 1
             if ip[2] > nz - 1: ip[2] = 0
-            nbr[q - 1, x] = ids.get(tuple(ip), -1) if solid[ip[0], ip[1], ip[2]] == 0 else -1
-    cells = np.array([(i * ny + j) * nz + k for (i, j, k, _) in order], np.int32)
-    return cells, n_int, np.array(tags, np.int32), nbr
+            nbr[q - 1, x] = ids[tuple(ip)] if solid[ip[0], ip[1], ip[2]] == 0 else -1
+    cells = np.array([(i * ny + j) * nz + k for (i, j, k) in order], np.int32)
+    return cells, np.array(rec, np.uint32), nbr

@@ -15,7 +15,7 @@ LIB_PATH = os.path.join(_HERE, "liblbm3d.so")
 SOURCES = [os.path.join(_HERE, "csrc", "lbm3d.hip")]
 INCLUDE = os.path.join(_ROOT, "include")
 
-ABI_VERSION = 1
+ABI_VERSION = 4
 F32, F16, BF16 = 0, 1, 2
 DTYPES = {"f32": F32, "f16": F16, "bf16": BF16}
 HIPCC_FLAGS = [
@@ -43,33 +43,40 @@ class Params(ctypes.Structure):
     """Mirror of struct lbm3d_params."""
     _fields_ = [
         ("nx", ctypes.c_int32), ("ny", ctypes.c_int32), ("nz", ctypes.c_int32),
-        ("n_cells", ctypes.c_int32), ("n_interior", ctypes.c_int32), ("force_on", ctypes.c_int32),
+        ("n_cells", ctypes.c_int32), ("force_on", ctypes.c_int32),
         ("s_v", ctypes.c_float), ("s_o", ctypes.c_float),
         ("fx", ctypes.c_float), ("fy", ctypes.c_float), ("fz", ctypes.c_float),
         ("guo_a", ctypes.c_float), ("guo_b", ctypes.c_float),
         ("bc_mode", ctypes.c_int32 * 6),
         ("rho_bc", ctypes.c_float * 6),
         ("u_bc", (ctypes.c_float * 3) * 6),
+        ("face_u_off", ctypes.c_int32 * 6),
     ]
 
 
 # name -> (restype, argtypes); every symbol include/lbm3d.h declares
-_vp, _i32, _i64, _sz = ctypes.c_void_p, ctypes.c_int32, ctypes.c_int64, ctypes.c_size_t
+_vp, _i32, _i64, _sz, _int = ctypes.c_void_p, ctypes.c_int32, ctypes.c_int64, ctypes.c_size_t, ctypes.c_int
 _pp = ctypes.POINTER(Params)
 SIGNATURES = {
-    "lbm3d_abi_version": (ctypes.c_int, []),
+    "lbm3d_abi_version": (_int, []),
     "lbm3d_last_error": (ctypes.c_char_p, []),
+    "lbm3d_slot_stride": (_i64, [_i32]),
+    "lbm3d_row_words": (_i64, [_pp]),
+    "lbm3d_face_bit_words": (_i64, [_i32]),
+    "lbm3d_face_plane": (_i64, [_pp, _i32]),
     "lbm3d_compact_workspace_bytes": (_sz, [_i64]),
-    "lbm3d_compact_count": (ctypes.c_int, [_vp, _pp, _vp, _vp, _vp, _sz]),
-    "lbm3d_compact_fill": (ctypes.c_int, [_vp, _pp, _vp, _vp, _vp, _vp, _vp, _sz]),
-    "lbm3d_build_neighbors": (ctypes.c_int, [_vp, _pp, _vp, _vp, _vp]),
-    "lbm3d_init": (ctypes.c_int, [_vp, _pp, ctypes.c_int, _vp, _vp, _vp, _vp]),
-    "lbm3d_step": (ctypes.c_int, [_vp, _pp, ctypes.c_int, _vp, _vp, _vp, _vp, _i64]),
-    "lbm3d_run": (ctypes.c_int, [_vp, _pp, ctypes.c_int, _vp, _vp, _vp, _vp, _i64, _i32]),
-    "lbm3d_macros": (ctypes.c_int, [_vp, _pp, ctypes.c_int, _vp, _vp, _vp, _vp, _i64, _vp, _vp]),
-    "lbm3d_scatter_dense": (ctypes.c_int, [_vp, _pp, _vp, _vp, _vp, _i32, _i32, _i32, _vp, _vp]),
+    "lbm3d_compact_count": (_int, [_vp, _pp, _vp, _vp, _vp, _sz]),
+    "lbm3d_compact_fill": (_int, [_vp, _pp, _vp, _vp, _vp, _vp, _sz]),
+    "lbm3d_build_rows": (_int, [_vp, _pp, _vp, _vp, _vp]),
+    "lbm3d_build_face_bits": (_int, [_vp, _pp, _vp, _vp, _vp]),
+    "lbm3d_build_neighbors": (_int, [_vp, _pp, _vp, _vp, _vp]),
+    "lbm3d_init": (_int, [_vp, _pp, _int, _vp, _vp, _vp, _vp, _vp]),
+    "lbm3d_step": (_int, [_vp, _pp, _int, _vp, _vp, _vp, _vp, _vp, _i64]),
+    "lbm3d_run": (_int, [_vp, _pp, _int, _vp, _vp, _vp, _vp, _vp, _i64, _i32]),
+    "lbm3d_macros": (_int, [_vp, _pp, _int, _vp, _vp, _vp, _vp, _vp, _i64, _vp, _vp]),
+    "lbm3d_scatter_dense": (_int, [_vp, _pp, _vp, _vp, _vp, _i32, _i32, _i32, _vp, _vp]),
     "lbm3d_reduce_workspace_bytes": (_sz, [_i32]),
-    "lbm3d_reduce": (ctypes.c_int, [_vp, _pp, _vp, _vp, _vp, _i32, _i32, _vp, _vp, _sz]),
+    "lbm3d_reduce": (_int, [_vp, _pp, _vp, _vp, _vp, _i32, _i32, _vp, _vp, _sz]),
 }
 
 

@@ -6,6 +6,16 @@
 // becomes one fused kernel over a compacted fluid-cell list, in place with the
 // AA access pattern, with the MRT transform held in registers.  The C ABI is
 // declared in include/lbm3d.h.
+//
+// Data in HBM (n fluid cells, C order, S = n rounded up to 64):
+//   g[19][S]       shifted populations f - w (FP32, FP16 or BF16)
+//   cells[n]       C-order dense index of each fluid cell
+//   rows[nx*ny][W] per grid row (i,j) and 32-cell word: fluid bit mask and the
+//                  compact index of the word's first fluid cell (8 bytes)
+//   face_bits      2 bits per cell: face-owned, owner's inward neighbour solid
+//   face_u         previous-step velocity of pressure-face nodes, per face plane
+// Neighbours are found through `rows` (compact index = base + popcount of the
+// mask below the cell), so no per-cell neighbour table is stored or read.
 #include <hip/hip_runtime.h>
 #include <hip/hip_fp16.h>
 #include <hip/hip_bf16.h>
@@ -19,7 +29,33 @@
 namespace {
 
 constexpr int kQ = 19;
-constexpr int kBlock = 256;
+#ifndef LBM3D_BLOCK
+#define LBM3D_BLOCK 256
+#endif
+constexpr int kBlock = LBM3D_BLOCK;  // threads per block, a multiple of 64
+
+// Build-time knobs (defaults are the measured best on MI355X).
+#ifndef LBM3D_XCD_REMAP_EVEN
+#define LBM3D_XCD_REMAP_EVEN 0  // contiguous block range per XCD on even steps
+#endif
+#ifndef LBM3D_XCD_REMAP_ODD
+#define LBM3D_XCD_REMAP_ODD 0   // ... and on odd steps
+#endif
+#ifndef LBM3D_STEP_MINB
+#define LBM3D_STEP_MINB 1
+#endif
+#ifndef LBM3D_REVERSE_EVEN
+// Even steps walk the block range backwards, so they first read the slots the
+// preceding odd step wrote last, while those are still in the 256 MB last-level
+// cache; the next odd step, walking forwards, likewise starts where this one ended.
+#define LBM3D_REVERSE_EVEN 1
+#endif
+#ifndef LBM3D_CPOL_LOAD
+#define LBM3D_CPOL_LOAD 0     // cache-policy bits of population loads
+#endif
+#ifndef LBM3D_CPOL_STORE
+#define LBM3D_CPOL_STORE 0    // cache-policy bits of population stores
+#endif
 
 thread_local char g_err[512] = "";
 
@@ -59,169 +95,400 @@ __host__ __device__ constexpr float wq(int q) {
   return q == 0 ? 1.0f / 3.0f : (q < 7 ? 1.0f / 18.0f : 1.0f / 36.0f);
 }
 
+// slot stride: n rounded up to whole 256-byte lines of FP32
+__host__ __device__ constexpr uint32_t slot_stride(uint32_t n) { return (n + 63u) & ~63u; }
+
 // ---------------------------------------------------------------------------
-// Storage conversion: populations live in HBM as g = f - w in T, compute in FP32.
+// Kernel parameters: the public struct plus values derived once on the host.
 // ---------------------------------------------------------------------------
+struct KP {
+  lbm3d_params p;
+  uint32_t S;              // slot stride
+  uint32_t W;              // 32-cell words per grid row
+  uint32_t nyz;            // ny * nz
+  uint32_t m_nz, s_nz;     // c / nz  == (c * m_nz) >> s_nz   for c < 2^31
+  uint32_t m_nyz, s_nyz;   // c / nyz == (c * m_nyz) >> s_nyz
+  uint32_t plane[6];       // cells per face plane
+  uint32_t n_row_words;    // nx * ny * W
+};
+
+// Exact division of 31-bit numerators by d: m = ceil(2^(31+l) / d), l = ceil(log2 d).
+void magic(uint32_t d, uint32_t &m, uint32_t &s) {
+  uint32_t l = 0;
+  while ((1ull << l) < d) ++l;
+  const uint64_t two = 1ull << (31 + l);
+  m = (uint32_t)((two + d - 1) / d);
+  s = 31 + l;
+}
+
+KP make_kp(const lbm3d_params &p) {
+  KP k;
+  k.p = p;
+  k.S = slot_stride((uint32_t)(p.n_cells > 0 ? p.n_cells : 0));
+  k.W = (uint32_t)(p.nz + 31) / 32;
+  k.nyz = (uint32_t)p.ny * (uint32_t)p.nz;
+  magic((uint32_t)p.nz, k.m_nz, k.s_nz);
+  magic(k.nyz, k.m_nyz, k.s_nyz);
+  k.plane[0] = k.plane[1] = (uint32_t)p.ny * p.nz;
+  k.plane[2] = k.plane[3] = (uint32_t)p.nx * p.nz;
+  k.plane[4] = k.plane[5] = (uint32_t)p.nx * p.ny;
+  k.n_row_words = (uint32_t)p.nx * (uint32_t)p.ny * k.W;
+  return k;
+}
+
+__device__ __forceinline__ uint32_t fdiv(uint32_t c, uint32_t m, uint32_t s) {
+  return (uint32_t)(((uint64_t)c * m) >> s);
+}
+
+__device__ __forceinline__ void coords(const KP &kp, uint32_t c, int &i, int &j, int &k) {
+  const uint32_t a = fdiv(c, kp.m_nyz, kp.s_nyz);
+  const uint32_t r = c - a * kp.nyz;
+  const uint32_t b = fdiv(r, kp.m_nz, kp.s_nz);
+  i = (int)a;
+  j = (int)b;
+  k = (int)(r - b * (uint32_t)kp.p.nz);
+}
+
+// periodic_index, SP:247-257: -1 -> n-1 and n -> 0 on every axis, whatever the BC mode
+__device__ __forceinline__ int wrap(int a, int n) { return a < 0 ? n - 1 : (a > n - 1 ? 0 : a); }
+
+// Block -> cell-range map.  Workgroups are dealt to the 8 XCDs round robin;
+// with the grid padded to a multiple of 8 the remap hands XCD k the k-th
+// contiguous eighth of the block range, so blocks sharing cache lines of a
+// slot share an L2.
+template <bool REMAP, bool REVERSE = false>
+__device__ __forceinline__ uint32_t logical_block() {
+  const uint32_t h = blockIdx.x;
+  if constexpr (REMAP) {
+    const uint32_t per = gridDim.x >> 3;
+    const uint32_t r = REVERSE ? per - 1u - (h >> 3) : (h >> 3);
+    return (h & 7u) * per + r;
+  } else {
+    return REVERSE ? gridDim.x - 1u - h : h;
+  }
+}
+
+inline unsigned grid_for(int64_t n) {
+  const int64_t b = (n + kBlock - 1) / kBlock;
+  return (unsigned)((b + 7) & ~int64_t(7));
+}
+
+// ---------------------------------------------------------------------------
+// Row lookup.  rows[(i*ny + j)*W + w] covers cells k = 32w .. 32w+31 of grid
+// row (i, j): bit b of .x set when cell 32w+b is fluid, .y the compact id of
+// the word's first fluid cell, .z / .w the compact ids (or -1 when not fluid)
+// of the cells just before the word's first cell and just after its last one,
+// with the periodic wrap of periodic_index (SP:247-257) at k = 0 and nz-1.
+// Fluid cells are numbered in C order, so inside a word the id of cell k is
+// .y plus the number of fluid cells below it.
+// ---------------------------------------------------------------------------
+struct Rows {
+  __amdgpu_buffer_rsrc_t r;
+  __device__ __forceinline__ Rows(const uint4 *rows, uint32_t n_words) {
+    r = __builtin_amdgcn_make_buffer_rsrc(const_cast<uint4 *>(rows), (short)0, (int)(n_words * 16u), 0x00020000);
+  }
+  __device__ __forceinline__ uint4 word(uint32_t row, uint32_t W, int k) const {
+    const uint32_t off = (row * W + ((uint32_t)k >> 5)) * 16u;
+    const auto v = __builtin_amdgcn_raw_buffer_load_b128(r, (int)off, 0, 0);
+    return make_uint4(v[0], v[1], v[2], v[3]);
+  }
+};
+
+__device__ __forceinline__ int32_t id_in_word(uint4 w, int k) {
+  const uint32_t b = (uint32_t)k & 31u;
+  const bool fl = (w.x >> b) & 1u;
+  return fl ? (int32_t)(w.y + __popc(w.x & ((1u << b) - 1u))) : -1;
+}
+
+// ids of cells (k-1, k, k+1) of one row, k-1 and k+1 wrapped, from one word
+__device__ __forceinline__ void row3(const Rows &rb, const KP &kp, uint32_t row, int k, int32_t &lo,
+                                     int32_t &mid, int32_t &hi) {
+  const uint4 w = rb.word(row, kp.W, k);
+  const uint32_t b = (uint32_t)k & 31u;
+  const uint32_t below = w.y + __popc(w.x & ((1u << b) - 1u));
+  const uint32_t fm = (w.x >> b) & 1u;
+  mid = fm ? (int32_t)below : -1;
+  const bool last = b == 31u || k == kp.p.nz - 1;
+  const bool up_fluid = (w.x >> ((b + 1u) & 31u)) & 1u;
+  hi = last ? (int32_t)w.w : (up_fluid ? (int32_t)(below + fm) : -1);
+  const bool down_fluid = (w.x >> ((b - 1u) & 31u)) & 1u;
+  lo = b == 0u ? (int32_t)w.z : (down_fluid ? (int32_t)below - 1 : -1);
+}
+
+// Compact ids of the 18 neighbours (index q-1), -1 where the neighbour is
+// not fluid: the push target of streaming1 (SP:264-266) or its bounce-back
+// (SP:267-268).
+__device__ __forceinline__ void neighbor_ids(const Rows &rb, const KP &kp, int i, int j, int k, int32_t nb[18]) {
+  const uint32_t ny = (uint32_t)kp.p.ny;
+  const uint32_t ip = (uint32_t)wrap(i + 1, kp.p.nx), im = (uint32_t)wrap(i - 1, kp.p.nx);
+  const uint32_t jp = (uint32_t)wrap(j + 1, kp.p.ny), jm = (uint32_t)wrap(j - 1, kp.p.ny);
+  const uint32_t ii = (uint32_t)i, jj = (uint32_t)j;
+  int32_t a, b, c;
+  // own row: q5 (0,0,1), q6 (0,0,-1)
+  row3(rb, kp, ii * ny + jj, k, a, b, c);
+  nb[4] = c;
+  nb[5] = a;
+  // row (i+1, j): q1 (1,0,0), q11 (1,0,1), q13 (1,0,-1)
+  row3(rb, kp, ip * ny + jj, k, a, b, c);
+  nb[0] = b;
+  nb[10] = c;
+  nb[12] = a;
+  // row (i-1, j): q2 (-1,0,0), q12 (-1,0,-1), q14 (-1,0,1)
+  row3(rb, kp, im * ny + jj, k, a, b, c);
+  nb[1] = b;
+  nb[11] = a;
+  nb[13] = c;
+  // row (i, j+1): q3 (0,1,0), q15 (0,1,1), q17 (0,1,-1)
+  row3(rb, kp, ii * ny + jp, k, a, b, c);
+  nb[2] = b;
+  nb[14] = c;
+  nb[16] = a;
+  // row (i, j-1): q4 (0,-1,0), q16 (0,-1,-1), q18 (0,-1,1)
+  row3(rb, kp, ii * ny + jm, k, a, b, c);
+  nb[3] = b;
+  nb[15] = a;
+  nb[17] = c;
+  // diagonal rows: q7 (1,1,0), q8 (-1,-1,0), q9 (1,-1,0), q10 (-1,1,0)
+  nb[6] = id_in_word(rb.word(ip * ny + jp, kp.W, k), k);
+  nb[7] = id_in_word(rb.word(im * ny + jm, kp.W, k), k);
+  nb[8] = id_in_word(rb.word(ip * ny + jm, kp.W, k), k);
+  nb[9] = id_in_word(rb.word(im * ny + jp, kp.W, k), k);
+}
+
+// ---------------------------------------------------------------------------
+// Population storage.  g = f - w in T (FP32, FP16 or BF16), arithmetic in FP32.
+// Every access goes through a buffer resource covering one opposite pair of
+// slots (directions 2k-1 and 2k; slot 0 alone): a wave-uniform descriptor in
+// SGPRs plus a 32-bit per-lane byte offset, so no per-access 64-bit address
+// arithmetic.  The descriptor's record count bounds every access.
+// ---------------------------------------------------------------------------
+constexpr int kBufferFlags = 0x00020000;  // dword 3 of a raw buffer resource on gfx9 parts
+
 template <typename T> struct St;
 template <> struct St<float> {
-  static __device__ __forceinline__ float ld(const float *p) { return *p; }
-  static __device__ __forceinline__ void st(float *p, float v) { *p = v; }
+  static __device__ __forceinline__ float ld(__amdgpu_buffer_rsrc_t r, uint32_t vo, uint32_t so) {
+    return __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(r, (int)vo, (int)so, LBM3D_CPOL_LOAD));
+  }
+  static __device__ __forceinline__ void st(__amdgpu_buffer_rsrc_t r, uint32_t vo, uint32_t so, float v) {
+    __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(v), r, (int)vo, (int)so, LBM3D_CPOL_STORE);
+  }
 };
 template <> struct St<__half> {
-  static __device__ __forceinline__ float ld(const __half *p) { return __half2float(*p); }
-  static __device__ __forceinline__ void st(__half *p, float v) { *p = __float2half_rn(v); }
+  static __device__ __forceinline__ float ld(__amdgpu_buffer_rsrc_t r, uint32_t vo, uint32_t so) {
+    const unsigned short b = __builtin_amdgcn_raw_buffer_load_b16(r, (int)vo, (int)so, 0);
+    return __half2float(__ushort_as_half(b));
+  }
+  static __device__ __forceinline__ void st(__amdgpu_buffer_rsrc_t r, uint32_t vo, uint32_t so, float v) {
+    __builtin_amdgcn_raw_buffer_store_b16(__half_as_ushort(__float2half_rn(v)), r, (int)vo, (int)so, 0);
+  }
 };
 template <> struct St<__hip_bfloat16> {
-  static __device__ __forceinline__ float ld(const __hip_bfloat16 *p) { return __bfloat162float(*p); }
-  static __device__ __forceinline__ void st(__hip_bfloat16 *p, float v) { *p = __float2bfloat16(v); }
+  static __device__ __forceinline__ float ld(__amdgpu_buffer_rsrc_t r, uint32_t vo, uint32_t so) {
+    const unsigned short b = __builtin_amdgcn_raw_buffer_load_b16(r, (int)vo, (int)so, 0);
+    return __uint_as_float((uint32_t)b << 16);
+  }
+  static __device__ __forceinline__ void st(__amdgpu_buffer_rsrc_t r, uint32_t vo, uint32_t so, float v) {
+    const __hip_bfloat16 h = __float2bfloat16(v);  // round to nearest even
+    __builtin_amdgcn_raw_buffer_store_b16(*reinterpret_cast<const unsigned short *>(&h), r, (int)vo, (int)so, 0);
+  }
+};
+
+// Resource k covers slots 2k-1 and 2k (k = 1..9); resource 0 covers slot 0.
+template <typename T>
+struct Slots {
+  __amdgpu_buffer_rsrc_t r[10];
+  uint32_t Sb;  // slot stride in bytes
+  __device__ __forceinline__ Slots(const T *g, uint32_t S) {
+    Sb = S * (uint32_t)sizeof(T);
+    char *base = (char *)const_cast<T *>(g);
+    r[0] = __builtin_amdgcn_make_buffer_rsrc(base, (short)0, (int)Sb, kBufferFlags);
+#pragma unroll
+    for (int k = 1; k <= 9; ++k)
+      r[k] = __builtin_amdgcn_make_buffer_rsrc(base + (size_t)(2 * k - 1) * Sb, (short)0, (int)(2 * Sb),
+                                               kBufferFlags);
+  }
+  // slot q at byte offset vo within the slot
+  __device__ __forceinline__ float ld_slot(int q, uint32_t vo) const {
+    return St<T>::ld(r[(q + 1) >> 1], vo, (q > 0 && !(q & 1)) ? Sb : 0u);
+  }
+  __device__ __forceinline__ void st_slot(int q, uint32_t vo, float v) const {
+    St<T>::st(r[(q + 1) >> 1], vo, (q > 0 && !(q & 1)) ? Sb : 0u, v);
+  }
+  // byte offset vo relative to the pair of direction q (vo < 2 Sb)
+  __device__ __forceinline__ float ld_pair(int q, uint32_t vo) const { return St<T>::ld(r[(q + 1) >> 1], vo, 0u); }
+  __device__ __forceinline__ void st_pair(int q, uint32_t vo, float v) const {
+    St<T>::st(r[(q + 1) >> 1], vo, 0u, v);
+  }
 };
 
 // ---------------------------------------------------------------------------
-// MRT collision on shifted populations g (g_q = f_q - w_q), in registers.
+// Collision.  The reference computes f = M^-1 [ m - S (m - meq) + (1 - S/2) G ]
+// with m = M F (SP:222-241).  Its relaxation vector S (SP:131) gives every
+// non-conserved moment of even parity (rows 1, 2, 9-15 of M, even functions of
+// e) the rate s_v and every non-conserved odd one (rows 4, 6, 8, 16-18) the
+// rate s_o, and its meq (SP:209-215) is exactly M feq(1, u) outside m0.  The
+// collision is therefore, direction by direction,
+//   f'_q = f_q - s_v (f+_q - E+_q) - s_o (f-_q - E-_q) + F_q,
+// f+ / f- the parts of f even / odd under e -> -e, E+_q = w_q (rho + 4.5 (e.u)^2
+// - 1.5 u.u), E-_q = 3 w_q e.j with j = sum e f (so the conserved m0 and j
+// relax by exactly nothing), and the force term split the same way:
+//   F_q = w_q [ (1 - s_v/2)(-a u.F + b (e.u)(e.F)) + (s_v/2) G0 + a e.F ],
+// G0 = -(3a - b)/3 u.F (a, b = 1/3, 1/9 in SP:236).  With g = f - w every E+
+// term loses its w_q and rho becomes drho = rho - 1.
 //
-// The reference computes f = M^-1 [ m - S (m - meq) + (1 - S/2) G ] with
-// m = M F (SP:226-241).  Here:
-//  * M is applied through opposite-pair sums P and differences D (even rows
-//    of M see only P, odd rows only D), every row being a polynomial in e:
-//    1, |e|^2-1, 3|e|^4-6|e|^2+1, e_x, (3|e|^2-5)e_x, ..., 3e_x^2-|e|^2, ...
-//  * M^-1 = M^T (M M^T)^-1 with M M^T block diagonal ({0,1,2}, {3,4}, {5,6},
-//    {7,8}, {9,10}, {11,12} and a diagonal), so the inverse is a few 2x2/3x3
-//    products followed by the integer M^T, again split into pair halves.
-//  * meq is the incompressible form of meq_vec (SP:209-215); it depends on u
-//    only for every moment but m0, whose rate S_0 is 0.  Because M w = e_0,
-//    shifting by w changes m0 alone (m0 = rho - 1), so the whole collision
-//    runs on g.
-//  * G_s = sum_l w_l [a (e_l - u).F + b (e_l.u)(e_l.F)] M[s,l] (SP:231-236)
-//    has the closed form below (nonzero only for s = 0,1,3,5,7,9,11,13,14,15).
+// Inputs: opposite-pair sums P[k] = g_lo + g_hi and differences D[k] = g_lo -
+// g_hi (k = 1..9, lo = 2k-1), rest population g0; outputs overwrite g.
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ void mrt_collide(float g[kQ], float ux, float uy, float uz,
-                                            const lbm3d_params &p) {
-  const float sv = p.s_v, so = p.s_o;
-  float P[10], D[10];
+struct Macro {
+  float drho, jx, jy, jz, ux, uy, uz;
+};
+
+// rho and v of streaming3 (SP:380-388): v = (j + F/2) / rho.  `first` drops
+// the half-force shift: init's state carries v = u0 exactly (SP:165-166).
+__device__ __forceinline__ Macro macros_pd(float g0, const float P[10], const float D[10], const lbm3d_params &p,
+                                           bool first) {
+  Macro m;
+  m.drho = g0 + ((P[1] + P[2]) + (P[3] + P[4])) + ((P[5] + P[6]) + (P[7] + P[8]) + P[9]);
+  // e_lo of the pairs: 1 (1,0,0) 2 (0,1,0) 3 (0,0,1) 4 (1,1,0) 5 (1,-1,0)
+  //                    6 (1,0,1) 7 (1,0,-1) 8 (0,1,1) 9 (0,1,-1)
+  m.jx = D[1] + (D[4] + D[5]) + (D[6] + D[7]);
+  m.jy = D[2] + (D[4] - D[5]) + (D[8] + D[9]);
+  m.jz = D[3] + (D[6] - D[7]) + (D[8] - D[9]);
+  const float inv = 1.0f / (1.0f + m.drho);
+  if (first) {
+    m.ux = m.jx * inv;
+    m.uy = m.jy * inv;
+    m.uz = m.jz * inv;
+  } else {
+    m.ux = (m.jx + 0.5f * p.fx) * inv;
+    m.uy = (m.jy + 0.5f * p.fy) * inv;
+    m.uz = (m.jz + 0.5f * p.fz) * inv;
+  }
+  return m;
+}
+
+// e_q . (x, y, z) with the zero and unit components folded at compile time
+// (q is a constant after unrolling; 0 * x is not 0 for every float, so the
+// products are not written out).
+__device__ __forceinline__ float edot(int q, float x, float y, float z) {
+  const int a = ex(q), b = ey(q), c = ez(q);
+  float r = 0.0f;
+  if (a != 0) r = a > 0 ? x : -x;
+  if (b != 0) r = (a != 0) ? (b > 0 ? r + y : r - y) : (b > 0 ? y : -y);
+  if (c != 0) r = (a != 0 || b != 0) ? (c > 0 ? r + z : r - z) : (c > 0 ? z : -z);
+  return r;
+}
+
+__device__ __forceinline__ void pair_sums(const float g[kQ], float P[10], float D[10]) {
 #pragma unroll
   for (int k = 1; k <= 9; ++k) {
     P[k] = g[2 * k - 1] + g[2 * k];
     D[k] = g[2 * k - 1] - g[2 * k];
   }
-  const float g0 = g[0];
-  const float s13 = P[1] + P[2] + P[3];
-  const float s49 = (P[4] + P[5]) + (P[6] + P[7]) + (P[8] + P[9]);
-  const float p4567 = (P[4] + P[5]) + (P[6] + P[7]);
-  const float p89 = P[8] + P[9];
+}
 
-  float m0 = g0 + s13 + s49;  // rho - 1
-  float m1 = s49 - g0;
-  float m2 = g0 - 2.0f * s13 + s49;
-  float m9 = 2.0f * P[1] - P[2] - P[3] + p4567 - 2.0f * p89;
-  float m10 = -2.0f * P[1] + P[2] + P[3] + p4567 - 2.0f * p89;
-  float m11 = P[2] - P[3] + (P[4] + P[5]) - (P[6] + P[7]);
-  float m12 = P[3] - P[2] + (P[4] + P[5]) - (P[6] + P[7]);
-  float m13 = P[4] - P[5];
-  float m14 = P[8] - P[9];
-  float m15 = P[6] - P[7];
-  const float dx4 = (D[4] + D[5]) + (D[6] + D[7]);
-  const float dy4 = (D[4] - D[5]) + (D[8] + D[9]);
-  const float dz4 = (D[6] - D[7]) + (D[8] - D[9]);
-  float m3 = D[1] + dx4;
-  float m4 = dx4 - 2.0f * D[1];
-  float m5 = D[2] + dy4;
-  float m6 = dy4 - 2.0f * D[2];
-  float m7 = D[3] + dz4;
-  float m8 = dz4 - 2.0f * D[3];
-  float m16 = (D[4] + D[5]) - (D[6] + D[7]);
-  float m17 = (D[5] - D[4]) + (D[8] + D[9]);
-  float m18 = (D[6] - D[7]) - (D[8] - D[9]);
-
-  // relaxation towards meq (SP:227-228)
-  const float uxx = ux * ux, uyy = uy * uy, uzz = uz * uz;
-  m1 -= sv * (m1 - (uxx + uyy + uzz));
-  m2 -= sv * m2;
-  m4 -= so * m4;
-  m6 -= so * m6;
-  m8 -= so * m8;
-  m9 -= sv * (m9 - (2.0f * uxx - uyy - uzz));
-  m10 -= sv * m10;
-  m11 -= sv * (m11 - (uyy - uzz));
-  m12 -= sv * m12;
-  m13 -= sv * (m13 - ux * uy);
-  m14 -= sv * (m14 - uy * uz);
-  m15 -= sv * (m15 - ux * uz);
-  m16 -= so * m16;
-  m17 -= so * m17;
-  m18 -= so * m18;
-
-  // forcing in moment space (SP:230-238), closed form of M g_guo
-  if (p.force_on) {
+template <bool FORCED>
+__device__ __forceinline__ void collide_impl(float g[kQ], const float P[10], const float D[10], const Macro &m,
+                                             const lbm3d_params &p) {
+  const float sv = p.s_v, so = p.s_o;
+  const float hsv = 0.5f * sv, hso = 0.5f * so;
+  const float ux = m.ux, uy = m.uy, uz = m.uz;
+  const float c0 = m.drho - 1.5f * (ux * ux + uy * uy + uz * uz);
+  float K0 = 0.f, K1 = 0.f, Fa = 0.f;
+  const float Fx = p.fx, Fy = p.fy, Fz = p.fz;
+  if constexpr (FORCED) {
     const float a = p.guo_a, b = p.guo_b;
-    const float Fx = p.fx, Fy = p.fy, Fz = p.fz;
     const float uF = ux * Fx + uy * Fy + uz * Fz;
-    const float hv = 1.0f - 0.5f * sv;
-    const float b29 = b * (2.0f / 9.0f), b9 = b * (1.0f / 9.0f);
-    m0 += -(3.0f * a - b) * (1.0f / 3.0f) * uF;
-    m1 += hv * b29 * uF;
-    m3 += a * (1.0f / 3.0f) * Fx;
-    m5 += a * (1.0f / 3.0f) * Fy;
-    m7 += a * (1.0f / 3.0f) * Fz;
-    m9 += hv * b29 * (2.0f * Fx * ux - Fy * uy - Fz * uz);
-    m11 += hv * b29 * (Fy * uy - Fz * uz);
-    m13 += hv * b9 * (Fx * uy + Fy * ux);
-    m14 += hv * b9 * (Fy * uz + Fz * uy);
-    m15 += hv * b9 * (Fx * uz + Fz * ux);
+    const float G0 = -(3.0f * a - b) * (1.0f / 3.0f) * uF;
+    K0 = -(1.0f - hsv) * a * uF + hsv * G0;
+    K1 = (1.0f - hsv) * b;
+    Fa = a;
   }
-
-  // y = (M M^T)^-1 m'
-  const float y0 = (5.0f / 36.0f) * m0 - (11.0f / 72.0f) * m1 + (1.0f / 24.0f) * m2;
-  const float y1 = -(11.0f / 72.0f) * m0 + (13.0f / 48.0f) * m1 - (11.0f / 144.0f) * m2;
-  const float y2 = (1.0f / 24.0f) * m0 - (11.0f / 144.0f) * m1 + (7.0f / 144.0f) * m2;
-  const float y3 = (1.0f / 9.0f) * m3 - (1.0f / 36.0f) * m4;
-  const float y4 = -(1.0f / 36.0f) * m3 + (5.0f / 72.0f) * m4;
-  const float y5 = (1.0f / 9.0f) * m5 - (1.0f / 36.0f) * m6;
-  const float y6 = -(1.0f / 36.0f) * m5 + (5.0f / 72.0f) * m6;
-  const float y7 = (1.0f / 9.0f) * m7 - (1.0f / 36.0f) * m8;
-  const float y8 = -(1.0f / 36.0f) * m7 + (5.0f / 72.0f) * m8;
-  const float y9 = (1.0f / 32.0f) * m9 - (1.0f / 96.0f) * m10;
-  const float y10 = -(1.0f / 96.0f) * m9 + (1.0f / 32.0f) * m10;
-  const float y11 = (3.0f / 32.0f) * m11 - (1.0f / 32.0f) * m12;
-  const float y12 = -(1.0f / 32.0f) * m11 + (3.0f / 32.0f) * m12;
-  const float y13 = 0.25f * m13, y14 = 0.25f * m14, y15 = 0.25f * m15;
-  const float y16 = 0.125f * m16, y17 = 0.125f * m17, y18 = 0.125f * m18;
-
-  // g' = M^T y, even half E and odd half O per opposite pair
-  const float A = y0 + y1 + y2, B = y9 + y10, C = y11 + y12;
-  const float a02 = y0 - 2.0f * y2;
-  const float b910 = y10 - y9;
-  float E[10], O[10];
-  E[1] = a02 - 2.0f * b910;
-  E[2] = a02 + b910 + (y11 - y12);
-  E[3] = a02 + b910 - (y11 - y12);
-  E[4] = A + B + C + y13;
-  E[5] = A + B + C - y13;
-  E[6] = A + B - C + y15;
-  E[7] = A + B - C - y15;
-  E[8] = A - 2.0f * B + y14;
-  E[9] = A - 2.0f * B - y14;
-  const float X = y3 + y4, Y = y5 + y6, Z = y7 + y8;
-  O[1] = y3 - 2.0f * y4;
-  O[2] = y5 - 2.0f * y6;
-  O[3] = y7 - 2.0f * y8;
-  O[4] = X + Y + (y16 - y17);
-  O[5] = X - Y + (y16 + y17);
-  O[6] = X + Z - (y16 - y18);
-  O[7] = X - Z - (y16 + y18);
-  O[8] = Y + Z + (y17 - y18);
-  O[9] = Y - Z + (y17 + y18);
-  g[0] = y0 - y1 + y2;
+  // rest direction
+  {
+    const float w = wq(0);
+    g[0] = g[0] - sv * (g[0] - w * c0);
+    if constexpr (FORCED) g[0] += w * K0;
+  }
 #pragma unroll
   for (int k = 1; k <= 9; ++k) {
-    g[2 * k - 1] = E[k] + O[k];
-    g[2 * k] = E[k] - O[k];
+    const int q = 2 * k - 1;
+    const float w = wq(q);
+    const float eu = edot(q, ux, uy, uz);
+    const float ej = edot(q, m.jx, m.jy, m.jz);
+    const float ap = hsv * P[k] - (sv * w) * (c0 + 4.5f * eu * eu);  // s_v (g+ - E+)
+    const float am = hso * D[k] - (so * 3.0f * w) * ej;              // s_o (g- - E-)
+    float lo = g[q] - ap - am, hi = g[q + 1] - ap + am;
+    if constexpr (FORCED) {
+      const float eF = edot(q, Fx, Fy, Fz);
+      const float fp = w * (K0 + K1 * eu * eF);
+      const float fm = (w * Fa) * eF;
+      lo += fp + fm;
+      hi += fp - fm;
+    }
+    g[q] = lo;
+    g[q + 1] = hi;
   }
 }
 
 // feq(q, rho, u) - w_q  (SP:152-158, shifted)
 __device__ __forceinline__ float feq_shift(int q, float rho, float ux, float uy, float uz, float uu) {
-  const float eu = (float)ex(q) * ux + (float)ey(q) * uy + (float)ez(q) * uz;
+  const float eu = edot(q, ux, uy, uz);
   return wq(q) * (rho - 1.0f) + wq(q) * rho * (3.0f * eu + 4.5f * eu * eu - 1.5f * uu);
+}
+
+// ---------------------------------------------------------------------------
+// Boundary faces.  Owner of a node = the last face, in the order
+// x-left, x-right, y-left, y-right, z-left, z-right, whose mode is 1 or 2 and
+// which contains the node (Boundary_condition applies faces in that order,
+// SP:274-370, so later faces overwrite earlier ones).
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ int face_owner(const lbm3d_params &p, int i, int j, int k) {
+  int owner = -1;
+#pragma unroll
+  for (int f = 0; f < 6; ++f) {
+    const int m = p.bc_mode[f];
+    if (m != 1 && m != 2) continue;
+    const int pos = f < 2 ? i : (f < 4 ? j : k);
+    const int n = f < 2 ? p.nx : (f < 4 ? p.ny : p.nz);
+    if (pos == ((f & 1) ? n - 1 : 0)) owner = f;
+  }
+  return owner;
+}
+
+// index of (i,j,k) in face f's plane
+__device__ __forceinline__ uint32_t plane_index(const lbm3d_params &p, int f, int i, int j, int k) {
+  return f < 2 ? (uint32_t)j * p.nz + k : (f < 4 ? (uint32_t)i * p.nz + k : (uint32_t)i * p.ny + j);
+}
+
+// Boundary_condition (SP:272-370) for a face-owned node: every population is
+// replaced by feq of the owner face.  Pressure (mode 1): rho_bc and the node's
+// previous-step velocity, or 0 when the inward neighbour is solid (its v is 0,
+// SP:390-392).  Velocity (mode 2): rho 1 and the face velocity.
+__device__ __forceinline__ void face_populations(float v[kQ], const KP &kp, int owner, bool inward_solid,
+                                                 const float *__restrict__ face_u, uint32_t pidx) {
+  const lbm3d_params &p = kp.p;
+  float rho, ux, uy, uz;
+  if (p.bc_mode[owner] == 1) {
+    rho = p.rho_bc[owner];
+    if (inward_solid) {
+      ux = uy = uz = 0.0f;
+    } else {
+      const float *fu = face_u + p.face_u_off[owner];
+      const uint32_t P = kp.plane[owner];
+      ux = fu[pidx];
+      uy = fu[P + pidx];
+      uz = fu[2 * P + pidx];
+    }
+  } else {
+    rho = 1.0f;
+    ux = p.u_bc[owner][0];
+    uy = p.u_bc[owner][1];
+    uz = p.u_bc[owner][2];
+  }
+  const float uu = ux * ux + uy * uy + uz * uz;
+#pragma unroll
+  for (int q = 0; q < kQ; ++q) v[q] = feq_shift(q, rho, ux, uy, uz, uu);
 }
 
 // ---------------------------------------------------------------------------
@@ -229,163 +496,139 @@ __device__ __forceinline__ float feq_shift(int q, float rho, float ux, float uy,
 //
 // Even t: cell x's populations sit in its own slots g[q][x].
 // Odd t : population q of cell x sits in slot opp(q) of the cell at x - e_q,
-//         or, when that cell is solid, in slot q of x itself (the bounce-back
-//         written by x on the previous step).  Equivalently, for each q the
-//         address  A_q = (nbr_q(x) >= 0) ? g[q][nbr_q(x)] : g[opp q][x]
-//         holds population opp(q); the step then writes post-collision f_q
-//         back to A_q, so every cell reads and writes the same set of slots
-//         and cells' sets are disjoint (race free in place).
+//         or, when that cell is not fluid, in slot q of x itself (the
+//         bounce-back x wrote on the previous step).  Equivalently, for each q
+//         the address  A_q = (nb_q(x) >= 0) ? g[q][nb_q(x)] : g[opp q][x]
+//         holds population opp(q); the step writes post-collision f_q back to
+//         A_q, so every cell reads and writes the same set of slots and cells'
+//         sets are disjoint (race free in place).  off[q] is A_q as a byte
+//         offset within q's slot pair.
 // ---------------------------------------------------------------------------
+struct CellCtx {
+  int i, j, k;
+  bool have_coords;
+};
+
 template <typename T, bool ODD>
-__device__ __forceinline__ void read_cell(const T *__restrict__ g, const int32_t *__restrict__ nbr,
-                                          uint32_t x, uint32_t N, float v[kQ], uint32_t off[kQ]) {
+__device__ __forceinline__ void read_cell(const Slots<T> &sl, const KP &kp, const int32_t *__restrict__ cells,
+                                          const Rows &rb, uint32_t x, float v[kQ], uint32_t off[kQ],
+                                          CellCtx &cc) {
+  const uint32_t xb = x * (uint32_t)sizeof(T);
   if constexpr (!ODD) {
+    cc.have_coords = false;
 #pragma unroll
-    for (int q = 0; q < kQ; ++q) v[q] = St<T>::ld(g + (size_t)q * N + x);
+    for (int q = 0; q < kQ; ++q) v[q] = sl.ld_slot(q, xb);
   } else {
-    off[0] = x;
+    coords(kp, (uint32_t)cells[x], cc.i, cc.j, cc.k);
+    cc.have_coords = true;
+    int32_t nb[18];
+    neighbor_ids(rb, kp, cc.i, cc.j, cc.k, nb);
+    off[0] = xb;
 #pragma unroll
     for (int q = 1; q < kQ; ++q) {
-      const int32_t c = nbr[(size_t)(q - 1) * N + x];
-      off[q] = c >= 0 ? (uint32_t)q * N + (uint32_t)c : (uint32_t)opp(q) * N + x;
+      const int32_t c = nb[q - 1];
+      const uint32_t self_slot = (q & 1) ? sl.Sb : 0u;  // slot opp(q) within the pair
+      const uint32_t nb_slot = (q & 1) ? 0u : sl.Sb;    // slot q within the pair
+      off[q] = c >= 0 ? nb_slot + (uint32_t)c * (uint32_t)sizeof(T) : self_slot + xb;
     }
-    v[0] = St<T>::ld(g + x);
+    v[0] = sl.ld_slot(0, xb);
 #pragma unroll
-    for (int q = 1; q < kQ; ++q) v[opp(q)] = St<T>::ld(g + off[q]);
+    for (int q = 1; q < kQ; ++q) v[opp(q)] = sl.ld_pair(q, off[q]);
   }
 }
 
-// Boundary_condition (SP:272-370) for a face-owned cell: every population is
-// replaced by feq of the owner face.  Pressure (mode 1): rho_bc and the cell's
-// previous-step velocity, or 0 when the inward neighbour is solid (its v is 0,
-// SP:390-392).  Velocity (mode 2): rho 1 and the face velocity.
-__device__ __forceinline__ void face_populations(float v[kQ], int32_t tag, const float *face_u,
-                                                 uint32_t fi, uint32_t nf, const lbm3d_params &p) {
-  const int face = LBM3D_TAG_FACE(tag);
-  const int mode = LBM3D_TAG_MODE(tag);
-  float rho, ux, uy, uz;
-  if (mode == 1) {
-    rho = p.rho_bc[face];
-    if (LBM3D_TAG_INWARD_SOLID(tag)) {
-      ux = uy = uz = 0.0f;
-    } else {
-      ux = face_u[fi];
-      uy = face_u[nf + fi];
-      uz = face_u[2 * nf + fi];
-    }
-  } else {
-    rho = 1.0f;
-    ux = p.u_bc[face][0];
-    uy = p.u_bc[face][1];
-    uz = p.u_bc[face][2];
-  }
-  const float uu = ux * ux + uy * uy + uz * uz;
-#pragma unroll
-  for (int q = 0; q < kQ; ++q) v[q] = feq_shift(q, rho, ux, uy, uz, uu);
+// 2-bit face record of cell x: bit 0 face-owned, bit 1 owner's inward neighbour solid
+__device__ __forceinline__ uint32_t face_record(const uint32_t *__restrict__ face_bits, uint32_t x) {
+  return (face_bits[x >> 4] >> ((x & 15u) * 2u)) & 3u;
 }
 
-// rho and v of streaming3 (SP:380-388) from shifted populations.  `first`
-// drops the half-force shift: the state left by init carries v = u0 exactly.
-__device__ __forceinline__ void macros_of(const float v[kQ], const lbm3d_params &p, bool first,
-                                          float &rho, float &ux, float &uy, float &uz) {
-  float dr = v[0];
-  float jx = 0.f, jy = 0.f, jz = 0.f;
-#pragma unroll
-  for (int k = 1; k <= 9; ++k) {
-    const float P = v[2 * k - 1] + v[2 * k];
-    const float D = v[2 * k - 1] - v[2 * k];
-    dr += P;
-    jx += (float)ex(2 * k - 1) * D;
-    jy += (float)ey(2 * k - 1) * D;
-    jz += (float)ez(2 * k - 1) * D;
-  }
-  rho = 1.0f + dr;
-  const float inv = 1.0f / rho;
-  if (first) {
-    ux = jx * inv;
-    uy = jy * inv;
-    uz = jz * inv;
-  } else {
-    ux = (jx + 0.5f * p.fx) * inv;
-    uy = (jy + 0.5f * p.fy) * inv;
-    uz = (jz + 0.5f * p.fz) * inv;
-  }
-}
-
-template <typename T, bool ODD>
-__global__ void __launch_bounds__(kBlock) k_step(const lbm3d_params p, T *__restrict__ g,
-                                                 const int32_t *__restrict__ nbr,
-                                                 const int32_t *__restrict__ face_tag,
-                                                 float *__restrict__ face_u, int first) {
-  const uint32_t x = blockIdx.x * kBlock + threadIdx.x;
-  const uint32_t N = (uint32_t)p.n_cells;
-  if (x >= N) return;
+template <typename T, bool ODD, bool FORCED>
+__global__ void __launch_bounds__(kBlock, LBM3D_STEP_MINB)
+    k_step(const KP kp, T *__restrict__ g, const int32_t *__restrict__ cells, const uint4 *__restrict__ rows,
+           const uint32_t *__restrict__ face_bits, float *__restrict__ face_u, int first) {
+  const uint32_t x = logical_block<ODD ? LBM3D_XCD_REMAP_ODD : LBM3D_XCD_REMAP_EVEN,
+                                   !ODD && LBM3D_REVERSE_EVEN>() * kBlock + threadIdx.x;
+  const lbm3d_params &p = kp.p;
+  if (x >= (uint32_t)p.n_cells) return;
+  const Slots<T> sl(g, kp.S);
+  const Rows rb(rows, kp.n_row_words);
   float v[kQ];
   uint32_t off[kQ];
-  read_cell<T, ODD>(g, nbr, x, N, v, off);
+  CellCtx cc;
+  read_cell<T, ODD>(sl, kp, cells, rb, x, v, off, cc);
 
-  const bool face = x >= (uint32_t)p.n_interior;
-  int32_t tag = 0;
-  uint32_t fi = 0;
-  const uint32_t nf = N - (uint32_t)p.n_interior;
-  if (face) {
-    fi = x - (uint32_t)p.n_interior;
-    tag = face_tag[fi];
-    if (!first) face_populations(v, tag, face_u, fi, nf, p);
+  const uint32_t fr = face_record(face_bits, x);
+  int owner = -1;
+  uint32_t pidx = 0;
+  if (fr & 1u) {
+    if (!cc.have_coords) coords(kp, (uint32_t)cells[x], cc.i, cc.j, cc.k);
+    owner = face_owner(p, cc.i, cc.j, cc.k);
+    pidx = plane_index(p, owner, cc.i, cc.j, cc.k);
+    if (!first) face_populations(v, kp, owner, (fr & 2u) != 0, face_u, pidx);
   }
-  float rho, ux, uy, uz;
-  macros_of(v, p, first != 0, rho, ux, uy, uz);
-  if (face && LBM3D_TAG_MODE(tag) == 1 && !LBM3D_TAG_INWARD_SOLID(tag)) {
+  float P[10], D[10];
+  pair_sums(v, P, D);
+  const Macro m = macros_pd(v[0], P, D, p, first != 0);
+  if (owner >= 0 && p.bc_mode[owner] == 1 && !(fr & 2u)) {
     // the pressure face of the next step uses this step's v (SP:281)
-    face_u[fi] = ux;
-    face_u[nf + fi] = uy;
-    face_u[2 * nf + fi] = uz;
+    float *fu = face_u + p.face_u_off[owner];
+    const uint32_t PL = kp.plane[owner];
+    fu[pidx] = m.ux;
+    fu[PL + pidx] = m.uy;
+    fu[2 * PL + pidx] = m.uz;
   }
-  mrt_collide(v, ux, uy, uz, p);
+  collide_impl<FORCED>(v, P, D, m, p);
 
   if constexpr (!ODD) {
+    const uint32_t xb = x * (uint32_t)sizeof(T);
 #pragma unroll
-    for (int q = 0; q < kQ; ++q) St<T>::st(g + (size_t)opp(q) * N + x, v[q]);
+    for (int q = 0; q < kQ; ++q) sl.st_slot(opp(q), xb, v[q]);
   } else {
-    St<T>::st(g + x, v[0]);
+    sl.st_slot(0, off[0], v[0]);
 #pragma unroll
-    for (int q = 1; q < kQ; ++q) St<T>::st(g + off[q], v[q]);
+    for (int q = 1; q < kQ; ++q) sl.st_pair(q, off[q], v[q]);
   }
 }
 
 template <typename T, bool ODD>
-__global__ void __launch_bounds__(kBlock) k_macros(const lbm3d_params p, const T *__restrict__ g,
-                                                   const int32_t *__restrict__ nbr,
-                                                   const int32_t *__restrict__ face_tag,
-                                                   const float *__restrict__ face_u, int first,
-                                                   float *__restrict__ rho_out,
-                                                   float *__restrict__ v_out) {
+__global__ void __launch_bounds__(kBlock)
+    k_macros(const KP kp, const T *__restrict__ g, const int32_t *__restrict__ cells,
+             const uint4 *__restrict__ rows, const uint32_t *__restrict__ face_bits,
+             const float *__restrict__ face_u, int first, float *__restrict__ rho_out, float *__restrict__ v_out) {
   const uint32_t x = blockIdx.x * kBlock + threadIdx.x;
+  const lbm3d_params &p = kp.p;
   const uint32_t N = (uint32_t)p.n_cells;
   if (x >= N) return;
+  const Slots<T> sl(g, kp.S);
+  const Rows rb(rows, kp.n_row_words);
   float v[kQ];
   uint32_t off[kQ];
-  read_cell<T, ODD>(g, nbr, x, N, v, off);
-  if (x >= (uint32_t)p.n_interior && !first) {
-    const uint32_t fi = x - (uint32_t)p.n_interior;
-    face_populations(v, face_tag[fi], face_u, fi, N - (uint32_t)p.n_interior, p);
+  CellCtx cc;
+  read_cell<T, ODD>(sl, kp, cells, rb, x, v, off, cc);
+  const uint32_t fr = face_record(face_bits, x);
+  if ((fr & 1u) && !first) {
+    if (!cc.have_coords) coords(kp, (uint32_t)cells[x], cc.i, cc.j, cc.k);
+    const int owner = face_owner(p, cc.i, cc.j, cc.k);
+    face_populations(v, kp, owner, (fr & 2u) != 0, face_u, plane_index(p, owner, cc.i, cc.j, cc.k));
   }
-  float rho, ux, uy, uz;
-  macros_of(v, p, first != 0, rho, ux, uy, uz);
-  rho_out[x] = rho;
-  v_out[x] = ux;
-  v_out[(size_t)N + x] = uy;
-  v_out[2 * (size_t)N + x] = uz;
+  float P[10], D[10];
+  pair_sums(v, P, D);
+  const Macro m = macros_pd(v[0], P, D, p, first != 0);
+  rho_out[x] = 1.0f + m.drho;
+  v_out[x] = m.ux;
+  v_out[(size_t)N + x] = m.uy;
+  v_out[2 * (size_t)N + x] = m.uz;
 }
 
 template <typename T>
-__global__ void __launch_bounds__(kBlock) k_init(const lbm3d_params p, T *__restrict__ g,
-                                                 float *__restrict__ face_u,
-                                                 const float *__restrict__ rho0,
-                                                 const float *__restrict__ u0) {
+__global__ void __launch_bounds__(kBlock)
+    k_init(const KP kp, T *__restrict__ g, const int32_t *__restrict__ cells, float *__restrict__ face_u,
+           const float *__restrict__ rho0, const float *__restrict__ u0) {
   const uint32_t x = blockIdx.x * kBlock + threadIdx.x;
+  const lbm3d_params &p = kp.p;
   const uint32_t N = (uint32_t)p.n_cells;
   if (x >= N) return;
+  const Slots<T> sl(g, kp.S);
   float rho = 1.0f, ux = 0.f, uy = 0.f, uz = 0.f;
   if (rho0) rho = rho0[x];
   if (u0) {
@@ -394,76 +637,44 @@ __global__ void __launch_bounds__(kBlock) k_init(const lbm3d_params p, T *__rest
     uz = u0[2 * (size_t)N + x];
   }
   const float uu = ux * ux + uy * uy + uz * uz;
+  const uint32_t xb = x * (uint32_t)sizeof(T);
 #pragma unroll
-  for (int q = 0; q < kQ; ++q) St<T>::st(g + (size_t)q * N + x, feq_shift(q, rho, ux, uy, uz, uu));
-  if (x >= (uint32_t)p.n_interior) {
-    const uint32_t fi = x - (uint32_t)p.n_interior, nf = N - (uint32_t)p.n_interior;
-    face_u[fi] = ux;
-    face_u[nf + fi] = uy;
-    face_u[2 * nf + fi] = uz;
+  for (int q = 0; q < kQ; ++q) sl.st_slot(q, xb, feq_shift(q, rho, ux, uy, uz, uu));
+  int i, j, k;
+  coords(kp, (uint32_t)cells[x], i, j, k);
+  const int owner = face_owner(p, i, j, k);
+  if (owner >= 0 && p.bc_mode[owner] == 1) {
+    float *fu = face_u + p.face_u_off[owner];
+    const uint32_t PL = kp.plane[owner], pi = plane_index(p, owner, i, j, k);
+    fu[pi] = ux;
+    fu[PL + pi] = uy;
+    fu[2 * PL + pi] = uz;
   }
 }
 
 // ---------------------------------------------------------------------------
-// Compaction.  Tiles of kTile cells (kItems per thread); per tile the counts
-// of interior and face-owned fluid cells, an exclusive scan over tiles, then a
-// block-level scan inside each tile to number the cells.
+// Compaction: tiles of kTile cells (kItems per thread); per tile the count of
+// fluid cells, an exclusive scan over tiles, then a block-level scan inside
+// each tile to number the cells in C order.
 // ---------------------------------------------------------------------------
 constexpr int kItems = 4;
 constexpr int kTile = kBlock * kItems;
 
-// class of dense cell c: 0 solid, 1 interior fluid, 2 face-owned fluid (tag set)
-__device__ __forceinline__ int classify(const lbm3d_params &p, const int8_t *__restrict__ solid,
-                                        int64_t c, int32_t &tag) {
-  if (solid[c] != 0) return 0;  // fluid iff solid == 0 (SP:225)
-  const int nyz = p.ny * p.nz;
-  const int i = (int)(c / nyz);
-  const int j = (int)((c / p.nz) % p.ny);
-  const int k = (int)(c % p.nz);
-  int owner = -1;
-#pragma unroll
-  for (int f = 0; f < 6; ++f) {
-    const int m = p.bc_mode[f];
-    if (m != 1 && m != 2) continue;
-    const int a = f >> 1;
-    const int pos = a == 0 ? i : (a == 1 ? j : k);
-    const int n = a == 0 ? p.nx : (a == 1 ? p.ny : p.nz);
-    if (pos == ((f & 1) ? n - 1 : 0)) owner = f;  // later faces overwrite earlier ones
-  }
-  if (owner < 0) return 1;
-  const int a = owner >> 1;
-  int ii = i, jj = j, kk = k;
-  const int n = a == 0 ? p.nx : (a == 1 ? p.ny : p.nz);
-  const int inw = (owner & 1) ? n - 2 : 1;
-  if (a == 0) ii = inw; else if (a == 1) jj = inw; else kk = inw;
-  const int inward_solid = solid[((int64_t)ii * p.ny + jj) * p.nz + kk] > 0 ? 1 : 0;  // SP:278
-  tag = owner | (p.bc_mode[owner] << 3) | (inward_solid << 5);
-  return 2;
-}
-
-__global__ void __launch_bounds__(kBlock) k_compact_count(const lbm3d_params p,
-                                                          const int8_t *__restrict__ solid,
-                                                          int64_t n_dense, int32_t *__restrict__ tile_cnt) {
-  __shared__ int s_cnt[2];
-  if (threadIdx.x == 0) s_cnt[0] = s_cnt[1] = 0;
+__global__ void __launch_bounds__(kBlock) k_compact_count(const int8_t *__restrict__ solid, int64_t n_dense,
+                                                          int32_t *__restrict__ tile_cnt) {
+  __shared__ int s_cnt;
+  if (threadIdx.x == 0) s_cnt = 0;
   __syncthreads();
   const int64_t base = (int64_t)blockIdx.x * kTile;
-  int ci = 0, cf = 0;
+  int cnt = 0;
   for (int it = 0; it < kItems; ++it) {
     const int64_t c = base + (int64_t)threadIdx.x * kItems + it;
     if (c >= n_dense) break;
-    int32_t tag;
-    const int cls = classify(p, solid, c, tag);
-    ci += cls == 1;
-    cf += cls == 2;
+    cnt += solid[c] == 0;  // fluid iff solid == 0 (SP:225)
   }
-  if (ci) atomicAdd(&s_cnt[0], ci);
-  if (cf) atomicAdd(&s_cnt[1], cf);
+  if (cnt) atomicAdd(&s_cnt, cnt);
   __syncthreads();
-  if (threadIdx.x == 0) {
-    tile_cnt[2 * blockIdx.x] = s_cnt[0];
-    tile_cnt[2 * blockIdx.x + 1] = s_cnt[1];
-  }
+  if (threadIdx.x == 0) tile_cnt[blockIdx.x] = s_cnt;
 }
 
 // inclusive scan of one int per thread across a kBlock-thread block
@@ -482,107 +693,127 @@ __device__ __forceinline__ int block_inclusive_scan(int val, int *s_warp) {
   return val + add;
 }
 
-// exclusive scan over tiles (single block, sequential chunks), totals to counts
+// exclusive scan over tiles (single block, sequential chunks), total to count[0]
 __global__ void __launch_bounds__(kBlock) k_compact_scan(int32_t *__restrict__ tile_cnt, int64_t n_tiles,
-                                                         int32_t *__restrict__ counts) {
+                                                         int32_t *__restrict__ count) {
   __shared__ int s_warp[kBlock / 64];
-  int carry_i = 0, carry_f = 0;
+  __shared__ int s_tot;
+  int carry = 0;
   for (int64_t start = 0; start < n_tiles; start += kBlock) {
     const int64_t t = start + threadIdx.x;
-    const int vi = t < n_tiles ? tile_cnt[2 * t] : 0;
-    const int vf = t < n_tiles ? tile_cnt[2 * t + 1] : 0;
-    const int si = block_inclusive_scan(vi, s_warp);
-    const int sf = block_inclusive_scan(vf, s_warp);
-    if (t < n_tiles) {
-      tile_cnt[2 * t] = carry_i + si - vi;
-      tile_cnt[2 * t + 1] = carry_f + sf - vf;
-    }
-    __shared__ int s_tot[2];
-    if (threadIdx.x == kBlock - 1) {
-      s_tot[0] = si;
-      s_tot[1] = sf;
-    }
+    const int v = t < n_tiles ? tile_cnt[t] : 0;
+    const int sc = block_inclusive_scan(v, s_warp);
+    if (t < n_tiles) tile_cnt[t] = carry + sc - v;
+    if (threadIdx.x == kBlock - 1) s_tot = sc;
     __syncthreads();
-    carry_i += s_tot[0];
-    carry_f += s_tot[1];
+    carry += s_tot;
     __syncthreads();
   }
-  if (threadIdx.x == 0) {
-    counts[0] = carry_i;
-    counts[1] = carry_f;
-  }
+  if (threadIdx.x == 0) count[0] = carry;
 }
 
-__global__ void __launch_bounds__(kBlock) k_compact_fill(const lbm3d_params p,
-                                                         const int8_t *__restrict__ solid,
-                                                         int64_t n_dense,
+__global__ void __launch_bounds__(kBlock) k_compact_fill(const int8_t *__restrict__ solid, int64_t n_dense,
                                                          const int32_t *__restrict__ tile_off,
-                                                         int32_t *__restrict__ d2c,
-                                                         int32_t *__restrict__ cells,
-                                                         int32_t *__restrict__ face_tag) {
+                                                         int32_t *__restrict__ d2c, int32_t *__restrict__ cells) {
   __shared__ int s_warp[kBlock / 64];
   const int64_t base = (int64_t)blockIdx.x * kTile;
-  int cls[kItems];
-  int32_t tags[kItems];
-  int ci = 0, cf = 0;
+  bool fl[kItems];
+  int cnt = 0;
 #pragma unroll
   for (int it = 0; it < kItems; ++it) {
     const int64_t c = base + (int64_t)threadIdx.x * kItems + it;
-    cls[it] = 0;
-    tags[it] = 0;
-    if (c < n_dense) cls[it] = classify(p, solid, c, tags[it]);
-    ci += cls[it] == 1;
-    cf += cls[it] == 2;
+    fl[it] = c < n_dense && solid[c] == 0;
+    cnt += fl[it];
   }
-  int oi = block_inclusive_scan(ci, s_warp) - ci + tile_off[2 * blockIdx.x];
-  int of = block_inclusive_scan(cf, s_warp) - cf + tile_off[2 * blockIdx.x + 1];
+  int o = block_inclusive_scan(cnt, s_warp) - cnt + tile_off[blockIdx.x];
 #pragma unroll
   for (int it = 0; it < kItems; ++it) {
     const int64_t c = base + (int64_t)threadIdx.x * kItems + it;
     if (c >= n_dense) break;
-    int32_t id = -1;
-    if (cls[it] == 1) {
-      id = oi++;
-    } else if (cls[it] == 2) {
-      face_tag[of] = tags[it];
-      id = p.n_interior + of++;
-    }
+    const int32_t id = fl[it] ? o++ : -1;
     if (d2c) d2c[c] = id;
     if (id >= 0) cells[id] = (int32_t)c;
   }
 }
 
-__global__ void __launch_bounds__(kBlock) k_neighbors(const lbm3d_params p, const int32_t *__restrict__ d2c,
-                                                      const int32_t *__restrict__ cells,
-                                                      int32_t *__restrict__ nbr) {
-  const uint32_t x = blockIdx.x * kBlock + threadIdx.x;
-  const uint32_t N = (uint32_t)p.n_cells;
-  if (x >= N) return;
-  const int64_t c = cells[x];
-  const int nyz = p.ny * p.nz;
-  const int i = (int)(c / nyz), j = (int)((c / p.nz) % p.ny), k = (int)(c % p.nz);
-#pragma unroll
-  for (int q = 1; q < kQ; ++q) {
-    // periodic_index, SP:247-257: -1 -> n-1, n -> 0 on every axis
-    int a = i + ex(q), b = j + ey(q), d = k + ez(q);
-    a = a < 0 ? p.nx - 1 : (a > p.nx - 1 ? 0 : a);
-    b = b < 0 ? p.ny - 1 : (b > p.ny - 1 ? 0 : b);
-    d = d < 0 ? p.nz - 1 : (d > p.nz - 1 ? 0 : d);
-    nbr[(size_t)(q - 1) * N + x] = d2c[((int64_t)a * p.ny + b) * p.nz + d];
+// rows[(i*ny + j)*W + w] = {fluid mask of cells 32w .. 32w+31, compact id of the
+// first of them, id of the cell before 32w, id of the cell after 32w+31}
+__global__ void __launch_bounds__(kBlock) k_build_rows(const KP kp, const int8_t *__restrict__ solid,
+                                                       const int32_t *__restrict__ d2c, uint4 *__restrict__ rows) {
+  const uint64_t t = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (t >= kp.n_row_words) return;
+  const uint64_t r = t / kp.W;
+  const uint32_t w = (uint32_t)(t % kp.W);
+  const int nz = kp.p.nz;
+  const int64_t row0 = (int64_t)r * nz;
+  const int64_t c0 = row0 + 32 * w;
+  uint32_t mask = 0;
+  int32_t base = 0;
+  bool found = false;
+  for (int b = 0; b < 32; ++b) {
+    if ((int)(32 * w) + b >= nz) break;
+    if (solid[c0 + b] == 0) {
+      mask |= 1u << b;
+      if (!found) {
+        base = d2c[c0 + b];
+        found = true;
+      }
+    }
   }
+  const int left = w == 0 ? nz - 1 : (int)(32 * w) - 1;            // wrap at k = 0
+  const int right = (w == kp.W - 1) ? 0 : (int)(32 * w) + 32;      // wrap after k = nz-1
+  rows[t] = make_uint4(mask, (uint32_t)base, (uint32_t)d2c[row0 + left], (uint32_t)d2c[row0 + right]);
 }
 
-__global__ void __launch_bounds__(kBlock) k_scatter(const lbm3d_params p, const int32_t *__restrict__ cells,
-                                                    const float *__restrict__ rho,
-                                                    const float *__restrict__ v, int32_t ox, int32_t oy,
-                                                    int32_t oz, float *__restrict__ rho_d,
+// face_bits: 16 cells per word, 2 bits per cell
+__global__ void __launch_bounds__(kBlock) k_build_face_bits(const KP kp, const int8_t *__restrict__ solid,
+                                                            const int32_t *__restrict__ cells,
+                                                            uint32_t *__restrict__ face_bits) {
+  const uint32_t wi = blockIdx.x * kBlock + threadIdx.x;
+  const lbm3d_params &p = kp.p;
+  const uint32_t N = (uint32_t)p.n_cells;
+  if (wi >= (N + 15) / 16) return;
+  uint32_t word = 0;
+  for (uint32_t b = 0; b < 16; ++b) {
+    const uint32_t x = wi * 16 + b;
+    if (x >= N) break;
+    int i, j, k;
+    coords(kp, (uint32_t)cells[x], i, j, k);
+    const int owner = face_owner(p, i, j, k);
+    if (owner < 0) continue;
+    int ii = i, jj = j, kk = k;
+    const int n = owner < 2 ? p.nx : (owner < 4 ? p.ny : p.nz);
+    const int inw = (owner & 1) ? n - 2 : 1;
+    if (owner < 2) ii = inw; else if (owner < 4) jj = inw; else kk = inw;
+    const bool inward_solid = solid[((int64_t)ii * p.ny + jj) * p.nz + kk] > 0;  // SP:278
+    word |= (1u | (inward_solid ? 2u : 0u)) << (2 * b);
+  }
+  face_bits[wi] = word;
+}
+
+__global__ void __launch_bounds__(kBlock) k_neighbors(const KP kp, const int32_t *__restrict__ cells,
+                                                      const uint4 *__restrict__ rows, int32_t *__restrict__ nbr) {
+  const uint32_t x = blockIdx.x * kBlock + threadIdx.x;
+  const uint32_t N = (uint32_t)kp.p.n_cells;
+  if (x >= N) return;
+  int i, j, k;
+  coords(kp, (uint32_t)cells[x], i, j, k);
+  int32_t nb[18];
+  const Rows rb(rows, kp.n_row_words);
+  neighbor_ids(rb, kp, i, j, k, nb);
+#pragma unroll
+  for (int q = 0; q < 18; ++q) nbr[(size_t)q * N + x] = nb[q];
+}
+
+__global__ void __launch_bounds__(kBlock) k_scatter(const KP kp, const int32_t *__restrict__ cells,
+                                                    const float *__restrict__ rho, const float *__restrict__ v,
+                                                    int32_t ox, int32_t oy, int32_t oz, float *__restrict__ rho_d,
                                                     float *__restrict__ v_d) {
   const uint32_t x = blockIdx.x * kBlock + threadIdx.x;
-  const uint32_t N = (uint32_t)p.n_cells;
+  const uint32_t N = (uint32_t)kp.p.n_cells;
   if (x >= N) return;
-  const int64_t c = cells[x];
-  const int nyz = p.ny * p.nz;
-  const int i = (int)(c / nyz), j = (int)((c / p.nz) % p.ny), k = (int)(c % p.nz);
+  int i, j, k;
+  coords(kp, (uint32_t)cells[x], i, j, k);
   const int64_t o = ((int64_t)i * oy + j) * oz + k;
   if (rho_d) rho_d[o] = rho[x];
   if (v_d) {
@@ -594,14 +825,12 @@ __global__ void __launch_bounds__(kBlock) k_scatter(const lbm3d_params p, const 
 
 // FP64 reductions: per-block partials, then one block folds them.
 constexpr int kRed = 8;
-__global__ void __launch_bounds__(kBlock) k_reduce_partial(const lbm3d_params p,
-                                                           const int32_t *__restrict__ cells,
+__global__ void __launch_bounds__(kBlock) k_reduce_partial(const KP kp, const int32_t *__restrict__ cells,
                                                            const float *__restrict__ rho,
                                                            const float *__restrict__ v, int32_t i_lo,
                                                            int32_t i_hi, double *__restrict__ part) {
-  const uint32_t N = (uint32_t)p.n_cells;
+  const uint32_t N = (uint32_t)kp.p.n_cells;
   double acc[kRed] = {0, 0, 0, 0, 0, 0, 0, 0};
-  const int nyz = p.ny * p.nz;
   for (uint32_t x = blockIdx.x * kBlock + threadIdx.x; x < N; x += gridDim.x * kBlock) {
     const float r = rho[x], a = v[x], b = v[(size_t)N + x], c = v[2 * (size_t)N + x];
     const bool fin = isfinite(r) && isfinite(a) && isfinite(b) && isfinite(c);
@@ -615,7 +844,7 @@ __global__ void __launch_bounds__(kBlock) k_reduce_partial(const lbm3d_params p,
     acc[3] += c;
     const double nrm = sqrt((double)a * a + (double)b * b + (double)c * c);
     acc[4] = nrm > acc[4] ? nrm : acc[4];
-    const int i = (int)(cells[x] / nyz);
+    const int i = (int)fdiv((uint32_t)cells[x], kp.m_nyz, kp.s_nyz);
     if (i >= i_lo && i <= i_hi) {
       acc[6] += a;
       acc[7] += 1.0;
@@ -641,8 +870,8 @@ __global__ void __launch_bounds__(kBlock) k_reduce_partial(const lbm3d_params p,
   }
 }
 
-__global__ void __launch_bounds__(kBlock) k_reduce_final(const double *__restrict__ part, int nparts,
-                                                         double *__restrict__ out) {
+__global__ void __launch_bounds__(64) k_reduce_final(const double *__restrict__ part, int nparts,
+                                                     double *__restrict__ out) {
   if (threadIdx.x >= kRed) return;
   const int r = threadIdx.x;
   double val = 0.0;
@@ -655,6 +884,9 @@ __global__ void __launch_bounds__(kBlock) k_reduce_final(const double *__restric
 
 constexpr int kRedBlocks = 1024;
 
+// ---------------------------------------------------------------------------
+// Host-side checks and launches.
+// ---------------------------------------------------------------------------
 int check_params(const lbm3d_params *p) {
   if (!p) return fail(LBM3D_EINVAL, "%s", "params is NULL");
   if (p->nx < 1 || p->ny < 1 || p->nz < 1) return fail(LBM3D_EINVAL, "%s", "grid extents must be >= 1");
@@ -670,58 +902,75 @@ int check_params(const lbm3d_params *p) {
 }
 
 int check_cells(const lbm3d_params *p) {
-  if (p->n_cells < 0 || p->n_interior < 0 || p->n_interior > p->n_cells)
-    return fail(LBM3D_EINVAL, "%s", "n_cells / n_interior out of range");
-  if ((uint64_t)p->n_cells * kQ > UINT32_MAX)
-    return fail(LBM3D_ESIZE, "%s", "n_cells * 19 exceeds the 32-bit slot offsets");
+  if (p->n_cells < 0 || (int64_t)p->n_cells > (int64_t)p->nx * p->ny * p->nz)
+    return fail(LBM3D_EINVAL, "%s", "n_cells out of range");
+  // a pair of FP32 slots is one buffer resource: its bytes must fit the 32-bit record count
+  if ((uint64_t)slot_stride((uint32_t)p->n_cells) * 2 * sizeof(float) > UINT32_MAX)
+    return fail(LBM3D_ESIZE, "%s", "two slots of n_cells exceed a 32-bit buffer range");
   return LBM3D_OK;
 }
 
-template <typename T>
-int launch_step(hipStream_t s, const lbm3d_params &p, void *g, const int32_t *nbr, const int32_t *tag,
-                float *face_u, int64_t t) {
-  const unsigned grid = (unsigned)((p.n_cells + kBlock - 1) / kBlock);
-  if (grid == 0) return LBM3D_OK;
-  if (t & 1)
-    k_step<T, true><<<grid, kBlock, 0, s>>>(p, (T *)g, nbr, tag, face_u, t == 0);
-  else
-    k_step<T, false><<<grid, kBlock, 0, s>>>(p, (T *)g, nbr, tag, face_u, t == 0);
-  return check_launch("lbm3d_step");
-}
-
-template <typename T>
-int launch_macros(hipStream_t s, const lbm3d_params &p, const void *g, const int32_t *nbr,
-                  const int32_t *tag, const float *face_u, int64_t t, float *rho, float *v) {
-  const unsigned grid = (unsigned)((p.n_cells + kBlock - 1) / kBlock);
-  if (grid == 0) return LBM3D_OK;
-  if (t & 1)
-    k_macros<T, true><<<grid, kBlock, 0, s>>>(p, (const T *)g, nbr, tag, face_u, t == 0, rho, v);
-  else
-    k_macros<T, false><<<grid, kBlock, 0, s>>>(p, (const T *)g, nbr, tag, face_u, t == 0, rho, v);
-  return check_launch("lbm3d_macros");
-}
-
-template <typename T>
-int launch_init(hipStream_t s, const lbm3d_params &p, void *g, float *face_u, const float *rho0,
-                const float *u0) {
-  const unsigned grid = (unsigned)((p.n_cells + kBlock - 1) / kBlock);
-  if (grid == 0) return LBM3D_OK;
-  k_init<T><<<grid, kBlock, 0, s>>>(p, (T *)g, face_u, rho0, u0);
-  return check_launch("lbm3d_init");
+int check_face_offsets(const lbm3d_params *p) {
+  for (int f = 0; f < 6; ++f)
+    if (p->bc_mode[f] == 1 && p->face_u_off[f] < 0)
+      return fail(LBM3D_EINVAL, "%s", "a pressure face has no face_u_off");
+  return LBM3D_OK;
 }
 
 bool valid_dtype(int d) { return d == LBM3D_F32 || d == LBM3D_F16 || d == LBM3D_BF16; }
 
-int solver_args(const lbm3d_params *p, int dtype, const void *g, const int32_t *nbr,
-                const int32_t *tag, const float *face_u) {
+int solver_args(const lbm3d_params *p, int dtype, const void *g, const int32_t *cells, const void *rows,
+                const uint32_t *face_bits) {
   int rc = check_params(p);
   if (rc) return rc;
   if ((rc = check_cells(p))) return rc;
+  if ((rc = check_face_offsets(p))) return rc;
   if (!valid_dtype(dtype)) return fail(LBM3D_EINVAL, "%s", "unknown dtype");
-  if (p->n_cells > 0 && (!g || !nbr)) return fail(LBM3D_EINVAL, "%s", "populations or neighbour table is NULL");
-  if (p->n_cells > p->n_interior && (!tag || !face_u))
-    return fail(LBM3D_EINVAL, "%s", "face cells present but face_tag or face_u is NULL");
+  if (p->n_cells > 0 && (!g || !cells || !rows || !face_bits))
+    return fail(LBM3D_EINVAL, "%s", "populations, cells, rows or face_bits is NULL");
   return LBM3D_OK;
+}
+
+template <typename T, bool FORCED>
+void launch_step_parity(hipStream_t s, const KP &kp, void *g, const int32_t *cells, const uint4 *rows,
+                        const uint32_t *face_bits, float *face_u, int64_t t) {
+  const unsigned grid = grid_for(kp.p.n_cells);
+  if (t & 1)
+    k_step<T, true, FORCED><<<grid, kBlock, 0, s>>>(kp, (T *)g, cells, rows, face_bits, face_u, t == 0);
+  else
+    k_step<T, false, FORCED><<<grid, kBlock, 0, s>>>(kp, (T *)g, cells, rows, face_bits, face_u, t == 0);
+}
+
+template <typename T>
+int launch_step(hipStream_t s, const KP &kp, void *g, const int32_t *cells, const uint4 *rows,
+                const uint32_t *face_bits, float *face_u, int64_t t) {
+  if (kp.p.n_cells == 0) return LBM3D_OK;
+  if (kp.p.force_on)
+    launch_step_parity<T, true>(s, kp, g, cells, rows, face_bits, face_u, t);
+  else
+    launch_step_parity<T, false>(s, kp, g, cells, rows, face_bits, face_u, t);
+  return check_launch("lbm3d_step");
+}
+
+template <typename T>
+int launch_macros(hipStream_t s, const KP &kp, const void *g, const int32_t *cells, const uint4 *rows,
+                  const uint32_t *face_bits, const float *face_u, int64_t t, float *rho, float *v) {
+  const unsigned grid = (unsigned)((kp.p.n_cells + kBlock - 1) / kBlock);
+  if (grid == 0) return LBM3D_OK;
+  if (t & 1)
+    k_macros<T, true><<<grid, kBlock, 0, s>>>(kp, (const T *)g, cells, rows, face_bits, face_u, t == 0, rho, v);
+  else
+    k_macros<T, false><<<grid, kBlock, 0, s>>>(kp, (const T *)g, cells, rows, face_bits, face_u, t == 0, rho, v);
+  return check_launch("lbm3d_macros");
+}
+
+template <typename T>
+int launch_init(hipStream_t s, const KP &kp, void *g, const int32_t *cells, float *face_u, const float *rho0,
+                const float *u0) {
+  const unsigned grid = (unsigned)((kp.p.n_cells + kBlock - 1) / kBlock);
+  if (grid == 0) return LBM3D_OK;
+  k_init<T><<<grid, kBlock, 0, s>>>(kp, (T *)g, cells, face_u, rho0, u0);
+  return check_launch("lbm3d_init");
 }
 
 }  // namespace
@@ -732,126 +981,178 @@ int lbm3d_abi_version(void) { return LBM3D_ABI_VERSION; }
 
 const char *lbm3d_last_error(void) { return g_err; }
 
-size_t lbm3d_compact_workspace_bytes(int64_t n_dense) {
-  const int64_t tiles = (n_dense + kTile - 1) / kTile;
-  return (size_t)(tiles > 0 ? tiles : 1) * 2 * sizeof(int32_t);
+int64_t lbm3d_slot_stride(int32_t n_cells) { return n_cells < 0 ? 0 : (int64_t)slot_stride((uint32_t)n_cells); }
+
+int64_t lbm3d_row_words(const lbm3d_params *p) {
+  if (!p || p->nx < 1 || p->ny < 1 || p->nz < 1) return 0;
+  return (int64_t)p->nx * p->ny * ((p->nz + 31) / 32);
 }
 
-int lbm3d_compact_count(void *stream, const lbm3d_params *p, const int8_t *solid, int32_t *counts,
-                        void *workspace, size_t workspace_bytes) {
+int64_t lbm3d_face_bit_words(int32_t n_cells) { return n_cells < 0 ? 0 : ((int64_t)n_cells + 15) / 16; }
+
+int64_t lbm3d_face_plane(const lbm3d_params *p, int32_t face) {
+  if (!p || face < 0 || face > 5) return 0;
+  return face < 2 ? (int64_t)p->ny * p->nz : (face < 4 ? (int64_t)p->nx * p->nz : (int64_t)p->nx * p->ny);
+}
+
+size_t lbm3d_compact_workspace_bytes(int64_t n_dense) {
+  const int64_t tiles = (n_dense + kTile - 1) / kTile;
+  return (size_t)(tiles > 0 ? tiles : 1) * sizeof(int32_t);
+}
+
+int lbm3d_compact_count(void *stream, const lbm3d_params *p, const int8_t *solid, int32_t *count, void *workspace,
+                        size_t workspace_bytes) {
   int rc = check_params(p);
   if (rc) return rc;
-  if (!solid || !counts || !workspace) return fail(LBM3D_EINVAL, "%s", "NULL argument to lbm3d_compact_count");
+  if (!solid || !count || !workspace) return fail(LBM3D_EINVAL, "%s", "NULL argument to lbm3d_compact_count");
   const int64_t n = (int64_t)p->nx * p->ny * p->nz;
   if (workspace_bytes < lbm3d_compact_workspace_bytes(n))
     return fail(LBM3D_EINVAL, "%s", "compaction workspace too small");
   const int64_t tiles = (n + kTile - 1) / kTile;
   hipStream_t s = (hipStream_t)stream;
-  k_compact_count<<<(unsigned)tiles, kBlock, 0, s>>>(*p, solid, n, (int32_t *)workspace);
+  k_compact_count<<<(unsigned)tiles, kBlock, 0, s>>>(solid, n, (int32_t *)workspace);
   if ((rc = check_launch("lbm3d_compact_count"))) return rc;
-  k_compact_scan<<<1, kBlock, 0, s>>>((int32_t *)workspace, tiles, counts);
+  k_compact_scan<<<1, kBlock, 0, s>>>((int32_t *)workspace, tiles, count);
   return check_launch("lbm3d_compact_count/scan");
 }
 
-int lbm3d_compact_fill(void *stream, const lbm3d_params *p, const int8_t *solid, int32_t *d2c,
-                       int32_t *cells, int32_t *face_tag, void *workspace, size_t workspace_bytes) {
+int lbm3d_compact_fill(void *stream, const lbm3d_params *p, const int8_t *solid, int32_t *d2c, int32_t *cells,
+                       void *workspace, size_t workspace_bytes) {
   int rc = check_params(p);
   if (rc) return rc;
   if ((rc = check_cells(p))) return rc;
-  if (!solid || !workspace || (p->n_cells > 0 && !cells) || (p->n_cells > p->n_interior && !face_tag))
+  if (!solid || !workspace || (p->n_cells > 0 && !cells))
     return fail(LBM3D_EINVAL, "%s", "NULL argument to lbm3d_compact_fill");
   const int64_t n = (int64_t)p->nx * p->ny * p->nz;
   if (workspace_bytes < lbm3d_compact_workspace_bytes(n))
     return fail(LBM3D_EINVAL, "%s", "compaction workspace too small");
   const int64_t tiles = (n + kTile - 1) / kTile;
-  k_compact_fill<<<(unsigned)tiles, kBlock, 0, (hipStream_t)stream>>>(*p, solid, n, (const int32_t *)workspace,
-                                                                      d2c, cells, face_tag);
+  k_compact_fill<<<(unsigned)tiles, kBlock, 0, (hipStream_t)stream>>>(solid, n, (const int32_t *)workspace, d2c,
+                                                                      cells);
   return check_launch("lbm3d_compact_fill");
 }
 
-int lbm3d_build_neighbors(void *stream, const lbm3d_params *p, const int32_t *d2c, const int32_t *cells,
+int lbm3d_build_rows(void *stream, const lbm3d_params *p, const int8_t *solid, const int32_t *d2c,
+                     lbm3d_row_word *rows) {
+  int rc = check_params(p);
+  if (rc) return rc;
+  if (!solid || !d2c || !rows) return fail(LBM3D_EINVAL, "%s", "NULL argument to lbm3d_build_rows");
+  const KP kp = make_kp(*p);
+  const int64_t n = lbm3d_row_words(p);
+  k_build_rows<<<(unsigned)((n + kBlock - 1) / kBlock), kBlock, 0, (hipStream_t)stream>>>(kp, solid, d2c,
+                                                                                         (uint4 *)rows);
+  return check_launch("lbm3d_build_rows");
+}
+
+int lbm3d_build_face_bits(void *stream, const lbm3d_params *p, const int8_t *solid, const int32_t *cells,
+                          uint32_t *face_bits) {
+  int rc = check_params(p);
+  if (rc) return rc;
+  if ((rc = check_cells(p))) return rc;
+  if (p->n_cells == 0) return LBM3D_OK;
+  if (!solid || !cells || !face_bits) return fail(LBM3D_EINVAL, "%s", "NULL argument to lbm3d_build_face_bits");
+  const KP kp = make_kp(*p);
+  const int64_t words = lbm3d_face_bit_words(p->n_cells);
+  k_build_face_bits<<<(unsigned)((words + kBlock - 1) / kBlock), kBlock, 0, (hipStream_t)stream>>>(kp, solid, cells,
+                                                                                                  face_bits);
+  return check_launch("lbm3d_build_face_bits");
+}
+
+int lbm3d_build_neighbors(void *stream, const lbm3d_params *p, const int32_t *cells, const lbm3d_row_word *rows,
                           int32_t *nbr) {
   int rc = check_params(p);
   if (rc) return rc;
   if ((rc = check_cells(p))) return rc;
   if (p->n_cells == 0) return LBM3D_OK;
-  if (!d2c || !cells || !nbr) return fail(LBM3D_EINVAL, "%s", "NULL argument to lbm3d_build_neighbors");
-  const unsigned grid = (unsigned)((p->n_cells + kBlock - 1) / kBlock);
-  k_neighbors<<<grid, kBlock, 0, (hipStream_t)stream>>>(*p, d2c, cells, nbr);
+  if (!cells || !rows || !nbr) return fail(LBM3D_EINVAL, "%s", "NULL argument to lbm3d_build_neighbors");
+  const KP kp = make_kp(*p);
+  k_neighbors<<<(unsigned)((p->n_cells + kBlock - 1) / kBlock), kBlock, 0, (hipStream_t)stream>>>(
+      kp, cells, (const uint4 *)rows, nbr);
   return check_launch("lbm3d_build_neighbors");
 }
 
-int lbm3d_init(void *stream, const lbm3d_params *p, int dtype, void *g, float *face_u, const float *rho0,
-               const float *u0) {
+int lbm3d_init(void *stream, const lbm3d_params *p, int dtype, void *g, const int32_t *cells, float *face_u,
+               const float *rho0, const float *u0) {
   int rc = check_params(p);
   if (rc) return rc;
   if ((rc = check_cells(p))) return rc;
+  if ((rc = check_face_offsets(p))) return rc;
   if (!valid_dtype(dtype)) return fail(LBM3D_EINVAL, "%s", "unknown dtype");
-  if (p->n_cells > 0 && !g) return fail(LBM3D_EINVAL, "%s", "populations are NULL");
-  if (p->n_cells > p->n_interior && !face_u) return fail(LBM3D_EINVAL, "%s", "face_u is NULL");
+  if (p->n_cells > 0 && (!g || !cells)) return fail(LBM3D_EINVAL, "%s", "populations or cells are NULL");
+  for (int f = 0; f < 6; ++f)
+    if (p->bc_mode[f] == 1 && !face_u) return fail(LBM3D_EINVAL, "%s", "face_u is NULL with a pressure face");
+  const KP kp = make_kp(*p);
   hipStream_t s = (hipStream_t)stream;
   switch (dtype) {
-    case LBM3D_F32: return launch_init<float>(s, *p, g, face_u, rho0, u0);
-    case LBM3D_F16: return launch_init<__half>(s, *p, g, face_u, rho0, u0);
-    default: return launch_init<__hip_bfloat16>(s, *p, g, face_u, rho0, u0);
+    case LBM3D_F32: return launch_init<float>(s, kp, g, cells, face_u, rho0, u0);
+    case LBM3D_F16: return launch_init<__half>(s, kp, g, cells, face_u, rho0, u0);
+    default: return launch_init<__hip_bfloat16>(s, kp, g, cells, face_u, rho0, u0);
   }
 }
 
-int lbm3d_step(void *stream, const lbm3d_params *p, int dtype, void *g, const int32_t *nbr,
-               const int32_t *face_tag, float *face_u, int64_t t) {
-  int rc = solver_args(p, dtype, g, nbr, face_tag, face_u);
+int lbm3d_step(void *stream, const lbm3d_params *p, int dtype, void *g, const int32_t *cells,
+               const lbm3d_row_word *rows, const uint32_t *face_bits, float *face_u, int64_t t) {
+  int rc = solver_args(p, dtype, g, cells, rows, face_bits);
   if (rc) return rc;
   if (t < 0) return fail(LBM3D_EINVAL, "%s", "negative step index");
+  const KP kp = make_kp(*p);
   hipStream_t s = (hipStream_t)stream;
+  const uint4 *r = (const uint4 *)rows;
   switch (dtype) {
-    case LBM3D_F32: return launch_step<float>(s, *p, g, nbr, face_tag, face_u, t);
-    case LBM3D_F16: return launch_step<__half>(s, *p, g, nbr, face_tag, face_u, t);
-    default: return launch_step<__hip_bfloat16>(s, *p, g, nbr, face_tag, face_u, t);
+    case LBM3D_F32: return launch_step<float>(s, kp, g, cells, r, face_bits, face_u, t);
+    case LBM3D_F16: return launch_step<__half>(s, kp, g, cells, r, face_bits, face_u, t);
+    default: return launch_step<__hip_bfloat16>(s, kp, g, cells, r, face_bits, face_u, t);
   }
 }
 
-int lbm3d_run(void *stream, const lbm3d_params *p, int dtype, void *g, const int32_t *nbr,
-              const int32_t *face_tag, float *face_u, int64_t t0, int32_t nsteps) {
-  int rc = solver_args(p, dtype, g, nbr, face_tag, face_u);
+int lbm3d_run(void *stream, const lbm3d_params *p, int dtype, void *g, const int32_t *cells,
+              const lbm3d_row_word *rows, const uint32_t *face_bits, float *face_u, int64_t t0, int32_t nsteps) {
+  int rc = solver_args(p, dtype, g, cells, rows, face_bits);
   if (rc) return rc;
   if (t0 < 0 || nsteps < 0) return fail(LBM3D_EINVAL, "%s", "negative step index or count");
+  const KP kp = make_kp(*p);
   hipStream_t s = (hipStream_t)stream;
+  const uint4 *r = (const uint4 *)rows;
   for (int32_t k = 0; k < nsteps; ++k) {
     switch (dtype) {
-      case LBM3D_F32: rc = launch_step<float>(s, *p, g, nbr, face_tag, face_u, t0 + k); break;
-      case LBM3D_F16: rc = launch_step<__half>(s, *p, g, nbr, face_tag, face_u, t0 + k); break;
-      default: rc = launch_step<__hip_bfloat16>(s, *p, g, nbr, face_tag, face_u, t0 + k); break;
+      case LBM3D_F32: rc = launch_step<float>(s, kp, g, cells, r, face_bits, face_u, t0 + k); break;
+      case LBM3D_F16: rc = launch_step<__half>(s, kp, g, cells, r, face_bits, face_u, t0 + k); break;
+      default: rc = launch_step<__hip_bfloat16>(s, kp, g, cells, r, face_bits, face_u, t0 + k); break;
     }
     if (rc) return rc;
   }
   return LBM3D_OK;
 }
 
-int lbm3d_macros(void *stream, const lbm3d_params *p, int dtype, const void *g, const int32_t *nbr,
-                 const int32_t *face_tag, const float *face_u, int64_t t, float *rho, float *v) {
-  int rc = solver_args(p, dtype, g, nbr, face_tag, face_u);
+int lbm3d_macros(void *stream, const lbm3d_params *p, int dtype, const void *g, const int32_t *cells,
+                 const lbm3d_row_word *rows, const uint32_t *face_bits, const float *face_u, int64_t t, float *rho,
+                 float *v) {
+  int rc = solver_args(p, dtype, g, cells, rows, face_bits);
   if (rc) return rc;
   if (t < 0) return fail(LBM3D_EINVAL, "%s", "negative step index");
   if (p->n_cells > 0 && (!rho || !v)) return fail(LBM3D_EINVAL, "%s", "rho or v output is NULL");
+  const KP kp = make_kp(*p);
   hipStream_t s = (hipStream_t)stream;
+  const uint4 *r = (const uint4 *)rows;
   switch (dtype) {
-    case LBM3D_F32: return launch_macros<float>(s, *p, g, nbr, face_tag, face_u, t, rho, v);
-    case LBM3D_F16: return launch_macros<__half>(s, *p, g, nbr, face_tag, face_u, t, rho, v);
-    default: return launch_macros<__hip_bfloat16>(s, *p, g, nbr, face_tag, face_u, t, rho, v);
+    case LBM3D_F32: return launch_macros<float>(s, kp, g, cells, r, face_bits, face_u, t, rho, v);
+    case LBM3D_F16: return launch_macros<__half>(s, kp, g, cells, r, face_bits, face_u, t, rho, v);
+    default: return launch_macros<__hip_bfloat16>(s, kp, g, cells, r, face_bits, face_u, t, rho, v);
   }
 }
 
 int lbm3d_scatter_dense(void *stream, const lbm3d_params *p, const int32_t *cells, const float *rho,
-                        const float *v, int32_t ox, int32_t oy, int32_t oz, float *rho_dense,
-                        float *v_dense) {
+                        const float *v, int32_t ox, int32_t oy, int32_t oz, float *rho_dense, float *v_dense) {
   int rc = check_params(p);
   if (rc) return rc;
   if ((rc = check_cells(p))) return rc;
   if (ox < p->nx || oy < p->ny || oz < p->nz) return fail(LBM3D_EINVAL, "%s", "dense extent smaller than the grid");
   if (p->n_cells == 0) return LBM3D_OK;
-  if (!cells || (rho_dense && !rho) || (v_dense && !v)) return fail(LBM3D_EINVAL, "%s", "NULL argument to lbm3d_scatter_dense");
+  if (!cells || (rho_dense && !rho) || (v_dense && !v))
+    return fail(LBM3D_EINVAL, "%s", "NULL argument to lbm3d_scatter_dense");
+  const KP kp = make_kp(*p);
   const unsigned grid = (unsigned)((p->n_cells + kBlock - 1) / kBlock);
-  k_scatter<<<grid, kBlock, 0, (hipStream_t)stream>>>(*p, cells, rho, v, ox, oy, oz, rho_dense, v_dense);
+  k_scatter<<<grid, kBlock, 0, (hipStream_t)stream>>>(kp, cells, rho, v, ox, oy, oz, rho_dense, v_dense);
   return check_launch("lbm3d_scatter_dense");
 }
 
@@ -871,8 +1172,9 @@ int lbm3d_reduce(void *stream, const lbm3d_params *p, const int32_t *cells, cons
     return fail(LBM3D_EINVAL, "%s", "reduction workspace too small");
   int blocks = (p->n_cells + kBlock - 1) / kBlock;
   blocks = blocks < 1 ? 1 : (blocks > kRedBlocks ? kRedBlocks : blocks);
+  const KP kp = make_kp(*p);
   hipStream_t s = (hipStream_t)stream;
-  k_reduce_partial<<<blocks, kBlock, 0, s>>>(*p, cells, rho, v, i_lo, i_hi, (double *)workspace);
+  k_reduce_partial<<<blocks, kBlock, 0, s>>>(kp, cells, rho, v, i_lo, i_hi, (double *)workspace);
   if ((rc = check_launch("lbm3d_reduce"))) return rc;
   k_reduce_final<<<1, 64, 0, s>>>((const double *)workspace, blocks, out);
   return check_launch("lbm3d_reduce/final");

@@ -261,6 +261,17 @@ class LB3D_Solver_Single_Phase:
                 raise ValueError(f"face {FACE_NAMES[f]} has a boundary condition but extent {n} < 2")
             p.bc_mode[f] = m
             p.rho_bc[f] = float(getattr(self, attrs[1]))
+        # state of each pressure face: 3 floats per node of its plane
+        off = 0
+        planes = (self.ny * self.nz, self.ny * self.nz, self.nx * self.nz, self.nx * self.nz,
+                  self.nx * self.ny, self.nx * self.ny)
+        for f in range(6):
+            if modes[f] == 1:
+                p.face_u_off[f] = off
+                off += 3 * planes[f]
+            else:
+                p.face_u_off[f] = -1
+        self._face_u_floats = off
         p.fx, p.fy, p.fz = float(self.fx), float(self.fy), float(self.fz)
         return modes
 
@@ -276,7 +287,7 @@ class LB3D_Solver_Single_Phase:
         return ctypes_void(torch.cuda.current_stream(self._torch_device()).cuda_stream)
 
     def _build_lists(self):
-        """Compacted fluid list, face tags and neighbour table on the GPU."""
+        """Fluid list (C order), row table and face bits on the GPU."""
         torch = _torch()
         lib = _lib.lib()
         dev = self._torch_device()
@@ -285,32 +296,45 @@ class LB3D_Solver_Single_Phase:
         solid = torch.from_numpy(np.ascontiguousarray(self.solid._data.reshape(-1))).to(dev)
         ws_bytes = lib.size("lbm3d_compact_workspace_bytes", n_dense)
         ws = torch.empty(ws_bytes, dtype=torch.uint8, device=dev)
-        counts = torch.zeros(2, dtype=torch.int32, device=dev)
+        count = torch.zeros(1, dtype=torch.int32, device=dev)
         with torch.cuda.device(dev):
             s = self._stream()
-            lib.call("lbm3d_compact_count", s, _ref(p), _ptr(solid), _ptr(counts), _ptr(ws), ws_bytes)
-            n_int, n_face = (int(x) for x in counts.cpu())
-            p.n_interior = n_int
-            p.n_cells = n_int + n_face
+            lib.call("lbm3d_compact_count", s, _ref(p), _ptr(solid), _ptr(count), _ptr(ws), ws_bytes)
+            p.n_cells = int(count.cpu()[0])
             d2c = torch.empty(n_dense, dtype=torch.int32, device=dev)
             cells = torch.empty(max(p.n_cells, 1), dtype=torch.int32, device=dev)
-            face_tag = torch.empty(max(n_face, 1), dtype=torch.int32, device=dev)
-            lib.call("lbm3d_compact_fill", s, _ref(p), _ptr(solid), _ptr(d2c), _ptr(cells), _ptr(face_tag),
-                     _ptr(ws), ws_bytes)
-            nbr = torch.empty(max(18 * p.n_cells, 1), dtype=torch.int32, device=dev)
-            lib.call("lbm3d_build_neighbors", s, _ref(p), _ptr(d2c), _ptr(cells), _ptr(nbr))
+            lib.call("lbm3d_compact_fill", s, _ref(p), _ptr(solid), _ptr(d2c), _ptr(cells), _ptr(ws), ws_bytes)
+            rows = torch.empty(max(4 * lib.size("lbm3d_row_words", _ref(p)), 4), dtype=torch.int32, device=dev)
+            lib.call("lbm3d_build_rows", s, _ref(p), _ptr(solid), _ptr(d2c), _ptr(rows))
+            face_bits = torch.zeros(max(lib.size("lbm3d_face_bit_words", p.n_cells), 1), dtype=torch.int32,
+                                    device=dev)
+            lib.call("lbm3d_build_face_bits", s, _ref(p), _ptr(solid), _ptr(cells), _ptr(face_bits))
+            torch.cuda.current_stream(dev).synchronize()
         del d2c, ws, solid
-        self._dev.update(cells=cells, face_tag=face_tag, nbr=nbr)
+        self._dev.update(cells=cells, rows=rows, face_bits=face_bits)
         self._modes_built = self._bc_modes()
+
+    def neighbor_table(self) -> np.ndarray:
+        """The explicit neighbour table (18, n_fluid), built on the GPU with the
+        lookup the step kernel uses: compact id of the cell at x + e_q, -1 if
+        that cell is not fluid (streaming1 bounce-back, SP:259-269)."""
+        torch = _torch()
+        p = self._params
+        dev = self._torch_device()
+        nbr = torch.empty(max(18 * p.n_cells, 1), dtype=torch.int32, device=dev)
+        with torch.cuda.device(dev):
+            _lib.lib().call("lbm3d_build_neighbors", self._stream(), _ref(p), _ptr(self._dev["cells"]),
+                            _ptr(self._dev["rows"]), _ptr(nbr))
+        return nbr[: 18 * p.n_cells].cpu().numpy().reshape(18, p.n_cells)
 
     def _alloc_state(self):
         torch = _torch()
         dev = self._torch_device()
         p = self._params
         tdt = {"f32": torch.float32, "f16": torch.float16, "bf16": torch.bfloat16}[self.dtype]
-        n_face = p.n_cells - p.n_interior
-        self._dev["g"] = torch.empty(max(19 * p.n_cells, 1), dtype=tdt, device=dev)
-        self._dev["face_u"] = torch.zeros(max(3 * n_face, 1), dtype=torch.float32, device=dev)
+        stride = _lib.lib().size("lbm3d_slot_stride", p.n_cells)
+        self._dev["g"] = torch.zeros(max(19 * stride, 1), dtype=tdt, device=dev)
+        self._dev["face_u"] = torch.zeros(max(self._face_u_floats, 1), dtype=torch.float32, device=dev)
 
     def _init_state(self, rho0=None, u0=None):
         """init kernel (SP:160-170): equilibrium at (rho0, u0), default rest state."""
@@ -320,8 +344,8 @@ class LB3D_Solver_Single_Phase:
         p = self._params
         with torch.cuda.device(dev):
             lib.call("lbm3d_init", self._stream(), _ref(p), _lib.DTYPES[self.dtype], _ptr(self._dev["g"]),
-                     _ptr(self._dev["face_u"]), _ptr(rho0) if rho0 is not None else None,
-                     _ptr(u0) if u0 is not None else None)
+                     _ptr(self._dev["cells"]), _ptr(self._dev["face_u"]),
+                     _ptr(rho0) if rho0 is not None else None, _ptr(u0) if u0 is not None else None)
         self._t = 0
         self._macro_cache = None
 
@@ -378,7 +402,8 @@ class LB3D_Solver_Single_Phase:
         d = self._dev
         with torch.cuda.device(self._torch_device()):
             _lib.lib().call("lbm3d_run", self._stream(), _ref(p), _lib.DTYPES[self.dtype], _ptr(d["g"]),
-                            _ptr(d["nbr"]), _ptr(d["face_tag"]), _ptr(d["face_u"]), self._t, int(nsteps))
+                            _ptr(d["cells"]), _ptr(d["rows"]), _ptr(d["face_bits"]), _ptr(d["face_u"]), self._t,
+                            int(nsteps))
         self._t += int(nsteps)
         self._macro_cache = None
 
@@ -406,8 +431,8 @@ class LB3D_Solver_Single_Phase:
         v = torch.empty(3 * n, dtype=torch.float32, device=dev)
         with torch.cuda.device(dev):
             _lib.lib().call("lbm3d_macros", self._stream(), _ref(p), _lib.DTYPES[self.dtype], _ptr(d["g"]),
-                            _ptr(d["nbr"]), _ptr(d["face_tag"]), _ptr(d["face_u"]), self._t, _ptr(rho),
-                            _ptr(v))
+                            _ptr(d["cells"]), _ptr(d["rows"]), _ptr(d["face_bits"]), _ptr(d["face_u"]), self._t,
+                            _ptr(rho), _ptr(v))
         self._macro_cache = (rho, v)
         return self._macro_cache
 
@@ -533,8 +558,7 @@ class LB3D_Solver_Single_Phase:
         """Populations (in their current AA parity), face velocities and step count."""
         torch = _torch()
         torch.save({"g": self._dev["g"].cpu(), "face_u": self._dev["face_u"].cpu(), "t": self._t,
-                    "dtype": self.dtype, "n_cells": self._params.n_cells,
-                    "n_interior": self._params.n_interior}, path)
+                    "dtype": self.dtype, "n_cells": self._params.n_cells}, path)
 
     def load_checkpoint(self, path: str):
         torch = _torch()

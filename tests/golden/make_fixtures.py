@@ -7,9 +7,10 @@ Inputs are the reference's own geometry data files (read as text):
       Single_phase/flow_domain_geo_generation_2D.py, one value per line, x fastest
   docs/geo.dat                -- 48^3 all-fluid mask
 They are stored bit-packed, in file order.  The compaction checksums are the
-SHA-256 of the fluid list and neighbour table that the plain-loop restatement
-(oracle.compaction.compact_loops) produces for the cavity mask with the
-example_cavity.py boundary (velocity on x-right, SP:405-407).
+SHA-256 of the fluid list and neighbour table, plus the face records, that
+the plain-loop restatement (oracle.compaction.compact_loops) produces for the
+cavity mask with the example_cavity.py boundary (velocity on x-right,
+SP:405-407).
 """
 import hashlib
 import os
@@ -38,12 +39,12 @@ def main():
     bits = np.unpackbits(d["bits"])[: int(d["n"])]
     solid = bits.reshape((50, 50, 50), order="F")
     modes = (0, 2, 0, 0, 0, 0)
-    cells, n_int, tags, nbr = compact_loops(solid, modes)
-    np.savez_compressed(os.path.join(HERE, "cavity_compaction.npz"), n_cells=cells.size, n_interior=n_int,
-                        modes=np.array(modes), face_tag=tags,
+    cells, rec, nbr = compact_loops(solid, modes)
+    np.savez_compressed(os.path.join(HERE, "cavity_compaction.npz"), n_cells=cells.size, modes=np.array(modes),
+                        face_rec=rec.astype(np.uint8),
                         sha_cells=hashlib.sha256(cells.astype("<i4").tobytes()).hexdigest(),
                         sha_nbr=hashlib.sha256(nbr.astype("<i4").tobytes()).hexdigest())
-    print("wrote fixtures:", cells.size, "fluid cells,", n_int, "interior")
+    print("wrote fixtures:", cells.size, "fluid cells,", int((rec > 0).sum()), "face-owned")
 
 
 if __name__ == "__main__":

@@ -54,33 +54,36 @@ def test_compaction_loops_equal_numpy(seed):
     # values of the i8 field: 0 fluid, > 0 solid, < 0 neither fluid nor "solid > 0"
     solid = rng.choice(np.array([0, 1, -1, 2], np.int8), size=shape, p=[0.7, 0.15, 0.1, 0.05])
     modes = tuple(int(x) for x in rng.integers(0, 3, 6))
-    cells, n_int, tags, d2c = oc.compact(solid, modes)
+    cells, d2c = oc.compact(solid)
     nbr = oc.neighbors(solid, cells, d2c)
-    c2, n2, t2, nb2 = oc.compact_loops(solid, modes)
+    rec = oc.face_records(solid, cells, modes)
+    c2, rec2, nb2 = oc.compact_loops(solid, modes)
     np.testing.assert_array_equal(cells, c2)
-    assert n_int == n2
-    np.testing.assert_array_equal(tags, t2)
+    np.testing.assert_array_equal(rec, rec2)
     np.testing.assert_array_equal(nbr, nb2)
+    words = oc.pack_face_bits(rec)
+    unpacked = (words[np.arange(rec.size) // 16] >> (2 * (np.arange(rec.size) % 16)).astype(np.uint32)) & 3
+    np.testing.assert_array_equal(unpacked, rec)
 
 
 def test_compaction_cavity_checksum():
     solid, _ = load_mask("geo_cavity.npz", (50, 50, 50))
     g = np.load(os.path.join(GOLDEN, "cavity_compaction.npz"))
-    cells, n_int, tags, d2c = oc.compact(solid, tuple(g["modes"]))
+    cells, d2c = oc.compact(solid)
     nbr = oc.neighbors(solid, cells, d2c)
-    assert cells.size == int(g["n_cells"]) == 112896 and n_int == int(g["n_interior"])
+    assert cells.size == int(g["n_cells"]) == 112896
     assert hashlib.sha256(cells.astype("<i4").tobytes()).hexdigest() == str(g["sha_cells"])
     assert hashlib.sha256(nbr.astype("<i4").tobytes()).hexdigest() == str(g["sha_nbr"])
-    np.testing.assert_array_equal(tags, g["face_tag"])
+    np.testing.assert_array_equal(oc.face_records(solid, cells, tuple(g["modes"])), g["face_rec"])
 
 
 def test_compaction_all_fluid_docs_fixture():
     """docs/geo.dat: 48^3 all fluid; every neighbour exists (periodic wrap, no bounce-back)."""
     solid, n_solid = load_mask("geo_docs48.npz", (48, 48, 48))
     assert n_solid == 0
-    cells, n_int, _, d2c = oc.compact(solid)
+    cells, d2c = oc.compact(solid)
     nbr = oc.neighbors(solid, cells, d2c)
-    assert cells.size == n_int == 48 ** 3 and nbr.min() >= 0
+    assert cells.size == 48 ** 3 and nbr.min() >= 0
     # each direction is a permutation of the cells
     for q in range(18):
         assert np.unique(nbr[q]).size == cells.size
@@ -98,7 +101,7 @@ def test_abi_exports_every_declared_symbol():
     from taichi_lbm3d_amd import _lib
     lib = _lib.lib()
     names = _header_functions()
-    assert len(names) >= 12
+    assert len(names) >= 18
     for n in names:
         assert hasattr(lib.cdll, n), n
         assert n in _lib.SIGNATURES, n
@@ -131,23 +134,36 @@ def test_abi_rejects_bad_arguments_without_gpu():
     lib = _lib.lib()
     p = _lib.Params()
     p.nx = p.ny = p.nz = 4
-    p.n_cells, p.n_interior = 10, 20
-    rc = lib.cdll.lbm3d_step(None, ctypes.byref(p), 0, None, None, None, None, 0)
-    assert rc == -1 and "n_interior" in lib.last_error()
-    p.n_cells = p.n_interior = 10
-    rc = lib.cdll.lbm3d_step(None, ctypes.byref(p), 7, ctypes.c_void_p(8), ctypes.c_void_p(8), None, None, 0)
+    vp = ctypes.c_void_p(8)
+    p.n_cells = 100
+    rc = lib.cdll.lbm3d_step(None, ctypes.byref(p), 0, vp, vp, vp, vp, vp, 0)
+    assert rc == -1 and "n_cells" in lib.last_error()
+    p.n_cells = 10
+    rc = lib.cdll.lbm3d_step(None, ctypes.byref(p), 7, vp, vp, vp, vp, vp, 0)
     assert rc == -1 and "dtype" in lib.last_error()
-    p.n_cells = 300_000_000
-    p.n_interior = p.n_cells
-    rc = lib.cdll.lbm3d_step(None, ctypes.byref(p), 0, ctypes.c_void_p(8), ctypes.c_void_p(8), None, None, 0)
+    rc = lib.cdll.lbm3d_step(None, ctypes.byref(p), 0, vp, None, vp, vp, vp, 0)
+    assert rc == -1 and "NULL" in lib.last_error()
+    p.bc_mode[0] = 1
+    p.face_u_off[0] = -1
+    rc = lib.cdll.lbm3d_step(None, ctypes.byref(p), 0, vp, vp, vp, vp, vp, 0)
+    assert rc == -1 and "face_u_off" in lib.last_error()
+    p.nx = p.ny = p.nz = 1000
+    p.bc_mode[0] = 0
+    p.n_cells = 600_000_000  # two FP32 slots beyond one 32-bit buffer range
+    rc = lib.cdll.lbm3d_step(None, ctypes.byref(p), 0, vp, vp, vp, vp, vp, 0)
     assert rc == -3
-    p.n_cells = p.n_interior = 10
+    p.nx = p.ny = p.nz = 4
+    p.n_cells = 10
     p.nx = 1
     p.bc_mode[0] = 1
-    rc = lib.cdll.lbm3d_compact_count(None, ctypes.byref(p), ctypes.c_void_p(8), ctypes.c_void_p(8),
-                                      ctypes.c_void_p(8), 1 << 20)
+    rc = lib.cdll.lbm3d_compact_count(None, ctypes.byref(p), vp, vp, vp, 1 << 20)
     assert rc == -1 and "extent" in lib.last_error()
-    assert lib.cdll.lbm3d_compact_workspace_bytes(2 * 1024) == 16
+    assert lib.cdll.lbm3d_compact_workspace_bytes(2 * 1024) == 8
+    p.nx, p.ny, p.nz = 5, 6, 70
+    assert lib.cdll.lbm3d_row_words(ctypes.byref(p)) == 5 * 6 * 3
+    assert lib.cdll.lbm3d_face_bit_words(33) == 3
+    assert [lib.cdll.lbm3d_face_plane(ctypes.byref(p), f) for f in range(6)] == [420, 420, 350, 350, 30, 30]
+    assert lib.cdll.lbm3d_slot_stride(65) == 128
 
 
 # ---- host logic ---------------------------------------------------------------

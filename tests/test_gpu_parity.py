@@ -80,46 +80,51 @@ def case_geometry(name):
 
 @pytest.mark.parametrize("name", sorted(CASES))
 def test_compaction_bit_exact(gpu, name):
+    """Fluid list, face records and the neighbour ids the step kernel computes
+    from the row table equal the brute-force restatement bit for bit."""
     import torch
     from taichi_lbm3d_amd import _lib
     solid = case_geometry(name)
     modes = CASES[name][4]
     s = make_solver(solid, CASES[name][2], CASES[name][3], modes, CASES[name][5], CASES[name][6])
-    cells, n_int, tags, d2c = oc.compact(solid, modes)
-    nbr = oc.neighbors(solid, cells, d2c)
-    p = s._params
-    assert p.n_interior == n_int and p.n_cells == cells.size
+    cells, d2c = oc.compact(solid)
+    assert s.n_fluid == cells.size
     np.testing.assert_array_equal(s._dev["cells"][: cells.size].cpu().numpy(), cells)
-    np.testing.assert_array_equal(s._dev["face_tag"][: tags.size].cpu().numpy(), tags)
-    np.testing.assert_array_equal(s._dev["nbr"][: 18 * cells.size].cpu().numpy().reshape(18, -1), nbr)
+    np.testing.assert_array_equal(s.neighbor_table(), oc.neighbors(solid, cells, d2c))
+    words = oc.pack_face_bits(oc.face_records(solid, cells, modes))
+    np.testing.assert_array_equal(s._dev["face_bits"][: words.size].cpu().numpy().view(np.uint32), words)
     # dense_to_compact through the raw C ABI
     lib = _lib.lib()
+    p = s._params
     n_dense = solid.size
-    sol = torch.from_numpy((solid != 0).astype(np.uint8).reshape(-1)).to(gpu)
+    sol = torch.from_numpy(solid.reshape(-1).copy()).to(gpu)
     ws_bytes = lib.size("lbm3d_compact_workspace_bytes", n_dense)
     ws = torch.empty(ws_bytes, dtype=torch.uint8, device=gpu)
     d2c_g = torch.empty(n_dense, dtype=torch.int32, device=gpu)
     cells_g = torch.empty(cells.size, dtype=torch.int32, device=gpu)
-    tag_g = torch.empty(max(tags.size, 1), dtype=torch.int32, device=gpu)
-    counts = torch.zeros(2, dtype=torch.int32, device=gpu)
+    count = torch.zeros(1, dtype=torch.int32, device=gpu)
     vp = lambda t: ctypes.c_void_p(t.data_ptr())
-    lib.call("lbm3d_compact_count", None, ctypes.byref(p), vp(sol), vp(counts), vp(ws), ws_bytes)
-    lib.call("lbm3d_compact_fill", None, ctypes.byref(p), vp(sol), vp(d2c_g), vp(cells_g), vp(tag_g), vp(ws),
-             ws_bytes)
+    lib.call("lbm3d_compact_count", None, ctypes.byref(p), vp(sol), vp(count), vp(ws), ws_bytes)
+    lib.call("lbm3d_compact_fill", None, ctypes.byref(p), vp(sol), vp(d2c_g), vp(cells_g), vp(ws), ws_bytes)
     torch.cuda.synchronize()
-    np.testing.assert_array_equal(counts.cpu().numpy(), [n_int, cells.size - n_int])
+    assert int(count.cpu()[0]) == cells.size
     np.testing.assert_array_equal(d2c_g.cpu().numpy(), d2c)
+    np.testing.assert_array_equal(cells_g.cpu().numpy(), cells)
 
 
 def test_compaction_cavity_fixture(gpu):
     """The reference's 50^3 cavity mask: 112,896 fluid cells (Single_phase/geo_cavity.dat)."""
+    import hashlib
+    g = np.load(os.path.join(os.path.dirname(__file__), "golden", "cavity_compaction.npz"))
     from taichi_lbm3d_amd.geometry import cavity
     solid = cavity(50)
     s = make_solver(solid, 0.1, (0, 0, 0), (0, 2, 0, 0, 0, 0), (1,) * 6, [(0, 0, 0), (0, 0, 0.1)] + [(0, 0, 0)] * 4)
-    cells, n_int, tags, d2c = oc.compact(solid, (0, 2, 0, 0, 0, 0))
-    assert s.n_fluid == 112896 == cells.size
-    np.testing.assert_array_equal(s._dev["cells"].cpu().numpy(), cells)
-    np.testing.assert_array_equal(s._dev["nbr"].cpu().numpy().reshape(18, -1), oc.neighbors(solid, cells, d2c))
+    assert s.n_fluid == 112896
+    cells = s._dev["cells"].cpu().numpy()
+    assert hashlib.sha256(cells.astype("<i4").tobytes()).hexdigest() == str(g["sha_cells"])
+    assert hashlib.sha256(s.neighbor_table().astype("<i4").tobytes()).hexdigest() == str(g["sha_nbr"])
+    words = oc.pack_face_bits(g["face_rec"].astype(np.uint32))
+    np.testing.assert_array_equal(s._dev["face_bits"][: words.size].cpu().numpy().view(np.uint32), words)
 
 
 @pytest.mark.parametrize("name", sorted(CASES))
