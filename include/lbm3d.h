@@ -20,8 +20,10 @@
  *   - Populations are stored shifted, g_q = f_q - w_q, in SoA order
  *     g[q * S + cell], q = 0..18 in the reference's direction order
  *     (SP:183-187), S = lbm3d_slot_stride(n_cells) (n_cells rounded up to a
- *     multiple of 64), in the storage type `dtype` (LBM3D_F32/F16/BF16), so
- *     g holds 19 * S elements; arithmetic is FP32.
+ *     multiple of 64), in the storage type `dtype` (LBM3D_F32/F16/BF16/F64),
+ *     so g holds 19 * S elements; arithmetic is FP32 (FP64 for LBM3D_F64).
+ *   - face_u (the pressure-face state) and the lbm3d_macros outputs are FP32,
+ *     or FP64 when dtype is LBM3D_F64.
  *   - Between steps g is in the AA pattern's layout for the step count t:
  *     even t, population q of a cell in its own slot q; odd t, in slot
  *     opp(q) of the upstream neighbour (or in the cell's own slot q where
@@ -37,12 +39,13 @@
 extern "C" {
 #endif
 
-#define LBM3D_ABI_VERSION 4
+#define LBM3D_ABI_VERSION 5
 
-/* storage types for populations */
+/* storage types for populations; F32, F16 and BF16 compute in FP32, F64 in FP64 */
 #define LBM3D_F32 0
 #define LBM3D_F16 1
 #define LBM3D_BF16 2
+#define LBM3D_F64 3
 
 /* error codes */
 #define LBM3D_OK 0
@@ -56,17 +59,21 @@ extern "C" {
 
 /* Frozen solver parameters: what SP bakes into its kernels at
  * init_simulation() and first compilation (SP:118-149, SP:230, SP:274-365). */
+/* Real parameters are doubles, as SP computes them in Python; a kernel rounds
+ * them once to its arithmetic type (FP32 for F32/F16/BF16 storage, as SP's
+ * f32 fields hold them, SP:54, :131). */
 typedef struct lbm3d_params {
   int32_t nx, ny, nz;      /* grid extent (SP:11) */
   int32_t n_cells;         /* fluid cells after compaction */
   int32_t force_on;        /* force_flag (SP:137-140) */
-  float s_v, s_o;          /* relaxation rates s_v and s_other (SP:127-129) */
-  float fx, fy, fz;        /* body force (SP:17, SP:134-136) */
-  float guo_a, guo_b;      /* force-term scalings: SP uses 1/3 and 1/9 (SP:236) */
   int32_t bc_mode[6];      /* 0 periodic, 1 fixed pressure, 2 fixed velocity (SP:22) */
-  float rho_bc[6];         /* rho_bcxl ... (SP:23-28) */
-  float u_bc[6][3];        /* vx_bcxl, vy_bcxl, vz_bcxl ... (SP:23-28) */
-  int32_t face_u_off[6];   /* float offset of each pressure face's state in face_u, -1 if unused */
+  int32_t face_u_off[6];   /* element offset of each pressure face's state in face_u, -1 if unused */
+  int32_t pad_;            /* keeps the doubles 8-byte aligned */
+  double s_v, s_o;         /* relaxation rates s_v and s_other (SP:127-129) */
+  double fx, fy, fz;       /* body force (SP:17, SP:134-136) */
+  double guo_a, guo_b;     /* force-term scalings: SP uses 1/3 and 1/9 (SP:236) */
+  double rho_bc[6];        /* rho_bcxl ... (SP:23-28) */
+  double u_bc[6][3];       /* vx_bcxl, vy_bcxl, vz_bcxl ... (SP:23-28) */
 } lbm3d_params;
 
 /* One 32-cell word of a grid row (i, j): bit b of `mask` set when cell
@@ -142,9 +149,9 @@ int lbm3d_build_neighbors(void *stream, const lbm3d_params *p, const int32_t *ce
 
 /* init (SP:160-170): g = feq(rho0, u0) - w on every fluid cell, and the
  * pressure-face state = u0 at the face nodes.  rho0 [n_cells] and u0 (SoA
- * [3][n_cells]) may be NULL for the reference's rest state rho = 1, u = 0. */
+ * [3][n_cells], FP32) may be NULL for the reference's rest state rho = 1, u = 0. */
 int lbm3d_init(void *stream, const lbm3d_params *p, int dtype, void *g, const int32_t *cells,
-               float *face_u, const float *rho0, const float *u0);
+               void *face_u, const float *rho0, const float *u0);
 
 /* One time step: the reference's colission -> streaming1 ->
  * Boundary_condition -> streaming3 (SP:477-481) as one fused kernel over the
@@ -153,18 +160,19 @@ int lbm3d_init(void *stream, const lbm3d_params *p, int dtype, void *g, const in
  * the step that starts from init's state (macros rho0, u0 without the
  * half-force shift, face populations as initialised). */
 int lbm3d_step(void *stream, const lbm3d_params *p, int dtype, void *g, const int32_t *cells,
-               const lbm3d_row_word *rows, const uint32_t *face_bits, float *face_u, int64_t t);
+               const lbm3d_row_word *rows, const uint32_t *face_bits, void *face_u, int64_t t);
 
 /* `nsteps` consecutive lbm3d_step calls from step t0, enqueued back to back. */
 int lbm3d_run(void *stream, const lbm3d_params *p, int dtype, void *g, const int32_t *cells,
-              const lbm3d_row_word *rows, const uint32_t *face_bits, float *face_u, int64_t t0,
+              const lbm3d_row_word *rows, const uint32_t *face_bits, void *face_u, int64_t t0,
               int32_t nsteps);
 
 /* Macroscopic fields after t steps, what SP's streaming3 leaves in rho and v
- * (SP:376-388): rho[n_cells] and v (SoA [3][n_cells]), FP32. */
+ * (SP:376-388): rho[n_cells] and v (SoA [3][n_cells]), FP32 (FP64 for
+ * LBM3D_F64).  lbm3d_scatter_dense and lbm3d_reduce take the FP32 form. */
 int lbm3d_macros(void *stream, const lbm3d_params *p, int dtype, const void *g, const int32_t *cells,
-                 const lbm3d_row_word *rows, const uint32_t *face_bits, const float *face_u, int64_t t,
-                 float *rho, float *v);
+                 const lbm3d_row_word *rows, const uint32_t *face_bits, const void *face_u, int64_t t,
+                 void *rho, void *v);
 
 /* Scatter compact rho/v into dense C-order arrays of extent (ox,oy,oz) >=
  * (nx,ny,nz): rho_dense[ox*oy*oz], v_dense[ox*oy*oz][3] (the to_numpy layout).

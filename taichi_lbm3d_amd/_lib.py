@@ -15,9 +15,9 @@ LIB_PATH = os.path.join(_HERE, "liblbm3d.so")
 SOURCES = [os.path.join(_HERE, "csrc", "lbm3d.hip")]
 INCLUDE = os.path.join(_ROOT, "include")
 
-ABI_VERSION = 4
-F32, F16, BF16 = 0, 1, 2
-DTYPES = {"f32": F32, "f16": F16, "bf16": BF16}
+ABI_VERSION = 5
+F32, F16, BF16, F64 = 0, 1, 2, 3
+DTYPES = {"f32": F32, "f16": F16, "bf16": BF16, "f64": F64}
 HIPCC_FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",
     # keep FP32 arithmetic unpacked (no v_pk_*_f32)
@@ -44,13 +44,14 @@ class Params(ctypes.Structure):
     _fields_ = [
         ("nx", ctypes.c_int32), ("ny", ctypes.c_int32), ("nz", ctypes.c_int32),
         ("n_cells", ctypes.c_int32), ("force_on", ctypes.c_int32),
-        ("s_v", ctypes.c_float), ("s_o", ctypes.c_float),
-        ("fx", ctypes.c_float), ("fy", ctypes.c_float), ("fz", ctypes.c_float),
-        ("guo_a", ctypes.c_float), ("guo_b", ctypes.c_float),
         ("bc_mode", ctypes.c_int32 * 6),
-        ("rho_bc", ctypes.c_float * 6),
-        ("u_bc", (ctypes.c_float * 3) * 6),
         ("face_u_off", ctypes.c_int32 * 6),
+        ("pad_", ctypes.c_int32),
+        ("s_v", ctypes.c_double), ("s_o", ctypes.c_double),
+        ("fx", ctypes.c_double), ("fy", ctypes.c_double), ("fz", ctypes.c_double),
+        ("guo_a", ctypes.c_double), ("guo_b", ctypes.c_double),
+        ("rho_bc", ctypes.c_double * 6),
+        ("u_bc", (ctypes.c_double * 3) * 6),
     ]
 
 

@@ -91,8 +91,9 @@ __host__ __device__ constexpr int ez(int q) {
   return t[q];
 }
 // weights, SP:195-197
-__host__ __device__ constexpr float wq(int q) {
-  return q == 0 ? 1.0f / 3.0f : (q < 7 ? 1.0f / 18.0f : 1.0f / 36.0f);
+template <typename R = float>
+__host__ __device__ constexpr R wq(int q) {
+  return q == 0 ? R(1) / R(3) : (q < 7 ? R(1) / R(18) : R(1) / R(36));
 }
 
 // slot stride: n rounded up to whole 256-byte lines of FP32
@@ -101,8 +102,18 @@ __host__ __device__ constexpr uint32_t slot_stride(uint32_t n) { return (n + 63u
 // ---------------------------------------------------------------------------
 // Kernel parameters: the public struct plus values derived once on the host.
 // ---------------------------------------------------------------------------
+// Real parameters in the kernel's arithmetic type, rounded once on the host.
+template <typename R>
+struct Phys {
+  R sv, so, fx, fy, fz, ga, gb;
+  R rho_bc[6];
+  R u_bc[6][3];
+};
+
 struct KP {
-  lbm3d_params p;
+  lbm3d_params p;          // integer fields are read on the device; its doubles are not
+  Phys<float> pf;
+  Phys<double> pd;
   uint32_t S;              // slot stride
   uint32_t W;              // 32-cell words per grid row
   uint32_t nyz;            // ny * nz
@@ -121,9 +132,26 @@ void magic(uint32_t d, uint32_t &m, uint32_t &s) {
   s = 31 + l;
 }
 
+template <typename R>
+void fill_phys(Phys<R> &f, const lbm3d_params &p) {
+  f.sv = (R)p.s_v;
+  f.so = (R)p.s_o;
+  f.fx = (R)p.fx;
+  f.fy = (R)p.fy;
+  f.fz = (R)p.fz;
+  f.ga = (R)p.guo_a;
+  f.gb = (R)p.guo_b;
+  for (int i = 0; i < 6; ++i) {
+    f.rho_bc[i] = (R)p.rho_bc[i];
+    for (int c = 0; c < 3; ++c) f.u_bc[i][c] = (R)p.u_bc[i][c];
+  }
+}
+
 KP make_kp(const lbm3d_params &p) {
   KP k;
   k.p = p;
+  fill_phys(k.pf, p);
+  fill_phys(k.pd, p);
   k.S = slot_stride((uint32_t)(p.n_cells > 0 ? p.n_cells : 0));
   k.W = (uint32_t)(p.nz + 31) / 32;
   k.nyz = (uint32_t)p.ny * (uint32_t)p.nz;
@@ -135,6 +163,10 @@ KP make_kp(const lbm3d_params &p) {
   k.n_row_words = (uint32_t)p.nx * (uint32_t)p.ny * k.W;
   return k;
 }
+
+template <typename R> __device__ __forceinline__ const Phys<R> &phys(const KP &kp);
+template <> __device__ __forceinline__ const Phys<float> &phys<float>(const KP &kp) { return kp.pf; }
+template <> __device__ __forceinline__ const Phys<double> &phys<double>(const KP &kp) { return kp.pd; }
 
 __device__ __forceinline__ uint32_t fdiv(uint32_t c, uint32_t m, uint32_t s) {
   return (uint32_t)(((uint64_t)c * m) >> s);
@@ -256,16 +288,18 @@ __device__ __forceinline__ void neighbor_ids(const Rows &rb, const KP &kp, int i
 }
 
 // ---------------------------------------------------------------------------
-// Population storage.  g = f - w in T (FP32, FP16 or BF16), arithmetic in FP32.
-// Every access goes through a buffer resource covering one opposite pair of
-// slots (directions 2k-1 and 2k; slot 0 alone): a wave-uniform descriptor in
-// SGPRs plus a 32-bit per-lane byte offset, so no per-access 64-bit address
-// arithmetic.  The descriptor's record count bounds every access.
+// Population storage.  g = f - w in T; FP32 arithmetic for FP32/FP16/BF16
+// storage, FP64 for FP64.  Every access goes through a buffer resource covering
+// one opposite pair of slots (directions 2k-1 and 2k; slot 0 alone): a
+// wave-uniform descriptor in SGPRs plus a 32-bit per-lane byte offset, so no
+// per-access 64-bit address arithmetic.  The descriptor's record count bounds
+// every access.
 // ---------------------------------------------------------------------------
 constexpr int kBufferFlags = 0x00020000;  // dword 3 of a raw buffer resource on gfx9 parts
 
 template <typename T> struct St;
 template <> struct St<float> {
+  using R = float;
   static __device__ __forceinline__ float ld(__amdgpu_buffer_rsrc_t r, uint32_t vo, uint32_t so) {
     return __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(r, (int)vo, (int)so, LBM3D_CPOL_LOAD));
   }
@@ -274,28 +308,47 @@ template <> struct St<float> {
   }
 };
 template <> struct St<__half> {
+  using R = float;
   static __device__ __forceinline__ float ld(__amdgpu_buffer_rsrc_t r, uint32_t vo, uint32_t so) {
-    const unsigned short b = __builtin_amdgcn_raw_buffer_load_b16(r, (int)vo, (int)so, 0);
+    const unsigned short b = __builtin_amdgcn_raw_buffer_load_b16(r, (int)vo, (int)so, LBM3D_CPOL_LOAD);
     return __half2float(__ushort_as_half(b));
   }
   static __device__ __forceinline__ void st(__amdgpu_buffer_rsrc_t r, uint32_t vo, uint32_t so, float v) {
-    __builtin_amdgcn_raw_buffer_store_b16(__half_as_ushort(__float2half_rn(v)), r, (int)vo, (int)so, 0);
+    __builtin_amdgcn_raw_buffer_store_b16(__half_as_ushort(__float2half_rn(v)), r, (int)vo, (int)so,
+                                          LBM3D_CPOL_STORE);
   }
 };
 template <> struct St<__hip_bfloat16> {
+  using R = float;
   static __device__ __forceinline__ float ld(__amdgpu_buffer_rsrc_t r, uint32_t vo, uint32_t so) {
-    const unsigned short b = __builtin_amdgcn_raw_buffer_load_b16(r, (int)vo, (int)so, 0);
+    const unsigned short b = __builtin_amdgcn_raw_buffer_load_b16(r, (int)vo, (int)so, LBM3D_CPOL_LOAD);
     return __uint_as_float((uint32_t)b << 16);
   }
   static __device__ __forceinline__ void st(__amdgpu_buffer_rsrc_t r, uint32_t vo, uint32_t so, float v) {
     const __hip_bfloat16 h = __float2bfloat16(v);  // round to nearest even
-    __builtin_amdgcn_raw_buffer_store_b16(*reinterpret_cast<const unsigned short *>(&h), r, (int)vo, (int)so, 0);
+    __builtin_amdgcn_raw_buffer_store_b16(*reinterpret_cast<const unsigned short *>(&h), r, (int)vo, (int)so,
+                                          LBM3D_CPOL_STORE);
+  }
+};
+template <> struct St<double> {
+  using R = double;
+  static __device__ __forceinline__ double ld(__amdgpu_buffer_rsrc_t r, uint32_t vo, uint32_t so) {
+    const auto v = __builtin_amdgcn_raw_buffer_load_b64(r, (int)vo, (int)so, LBM3D_CPOL_LOAD);
+    return __hiloint2double((int)v[1], (int)v[0]);
+  }
+  static __device__ __forceinline__ void st(__amdgpu_buffer_rsrc_t r, uint32_t vo, uint32_t so, double v) {
+    const uint64_t b = (uint64_t)__double_as_longlong(v);
+    decltype(__builtin_amdgcn_raw_buffer_load_b64(r, 0, 0, 0)) w;
+    w[0] = (uint32_t)b;
+    w[1] = (uint32_t)(b >> 32);
+    __builtin_amdgcn_raw_buffer_store_b64(w, r, (int)vo, (int)so, LBM3D_CPOL_STORE);
   }
 };
 
 // Resource k covers slots 2k-1 and 2k (k = 1..9); resource 0 covers slot 0.
 template <typename T>
 struct Slots {
+  using R = typename St<T>::R;
   __amdgpu_buffer_rsrc_t r[10];
   uint32_t Sb;  // slot stride in bytes
   __device__ __forceinline__ Slots(const T *g, uint32_t S) {
@@ -308,17 +361,15 @@ struct Slots {
                                                kBufferFlags);
   }
   // slot q at byte offset vo within the slot
-  __device__ __forceinline__ float ld_slot(int q, uint32_t vo) const {
+  __device__ __forceinline__ R ld_slot(int q, uint32_t vo) const {
     return St<T>::ld(r[(q + 1) >> 1], vo, (q > 0 && !(q & 1)) ? Sb : 0u);
   }
-  __device__ __forceinline__ void st_slot(int q, uint32_t vo, float v) const {
+  __device__ __forceinline__ void st_slot(int q, uint32_t vo, R v) const {
     St<T>::st(r[(q + 1) >> 1], vo, (q > 0 && !(q & 1)) ? Sb : 0u, v);
   }
   // byte offset vo relative to the pair of direction q (vo < 2 Sb)
-  __device__ __forceinline__ float ld_pair(int q, uint32_t vo) const { return St<T>::ld(r[(q + 1) >> 1], vo, 0u); }
-  __device__ __forceinline__ void st_pair(int q, uint32_t vo, float v) const {
-    St<T>::st(r[(q + 1) >> 1], vo, 0u, v);
-  }
+  __device__ __forceinline__ R ld_pair(int q, uint32_t vo) const { return St<T>::ld(r[(q + 1) >> 1], vo, 0u); }
+  __device__ __forceinline__ void st_pair(int q, uint32_t vo, R v) const { St<T>::st(r[(q + 1) >> 1], vo, 0u, v); }
 };
 
 // ---------------------------------------------------------------------------
@@ -339,47 +390,26 @@ struct Slots {
 // Inputs: opposite-pair sums P[k] = g_lo + g_hi and differences D[k] = g_lo -
 // g_hi (k = 1..9, lo = 2k-1), rest population g0; outputs overwrite g.
 // ---------------------------------------------------------------------------
+template <typename R>
 struct Macro {
-  float drho, jx, jy, jz, ux, uy, uz;
+  R drho, jx, jy, jz, ux, uy, uz;
 };
-
-// rho and v of streaming3 (SP:380-388): v = (j + F/2) / rho.  `first` drops
-// the half-force shift: init's state carries v = u0 exactly (SP:165-166).
-__device__ __forceinline__ Macro macros_pd(float g0, const float P[10], const float D[10], const lbm3d_params &p,
-                                           bool first) {
-  Macro m;
-  m.drho = g0 + ((P[1] + P[2]) + (P[3] + P[4])) + ((P[5] + P[6]) + (P[7] + P[8]) + P[9]);
-  // e_lo of the pairs: 1 (1,0,0) 2 (0,1,0) 3 (0,0,1) 4 (1,1,0) 5 (1,-1,0)
-  //                    6 (1,0,1) 7 (1,0,-1) 8 (0,1,1) 9 (0,1,-1)
-  m.jx = D[1] + (D[4] + D[5]) + (D[6] + D[7]);
-  m.jy = D[2] + (D[4] - D[5]) + (D[8] + D[9]);
-  m.jz = D[3] + (D[6] - D[7]) + (D[8] - D[9]);
-  const float inv = 1.0f / (1.0f + m.drho);
-  if (first) {
-    m.ux = m.jx * inv;
-    m.uy = m.jy * inv;
-    m.uz = m.jz * inv;
-  } else {
-    m.ux = (m.jx + 0.5f * p.fx) * inv;
-    m.uy = (m.jy + 0.5f * p.fy) * inv;
-    m.uz = (m.jz + 0.5f * p.fz) * inv;
-  }
-  return m;
-}
 
 // e_q . (x, y, z) with the zero and unit components folded at compile time
 // (q is a constant after unrolling; 0 * x is not 0 for every float, so the
 // products are not written out).
-__device__ __forceinline__ float edot(int q, float x, float y, float z) {
+template <typename R>
+__device__ __forceinline__ R edot(int q, R x, R y, R z) {
   const int a = ex(q), b = ey(q), c = ez(q);
-  float r = 0.0f;
+  R r = R(0);
   if (a != 0) r = a > 0 ? x : -x;
   if (b != 0) r = (a != 0) ? (b > 0 ? r + y : r - y) : (b > 0 ? y : -y);
   if (c != 0) r = (a != 0 || b != 0) ? (c > 0 ? r + z : r - z) : (c > 0 ? z : -z);
   return r;
 }
 
-__device__ __forceinline__ void pair_sums(const float g[kQ], float P[10], float D[10]) {
+template <typename R>
+__device__ __forceinline__ void pair_sums(const R g[kQ], R P[10], R D[10]) {
 #pragma unroll
   for (int k = 1; k <= 9; ++k) {
     P[k] = g[2 * k - 1] + g[2 * k];
@@ -387,42 +417,66 @@ __device__ __forceinline__ void pair_sums(const float g[kQ], float P[10], float 
   }
 }
 
-template <bool FORCED>
-__device__ __forceinline__ void collide_impl(float g[kQ], const float P[10], const float D[10], const Macro &m,
-                                             const lbm3d_params &p) {
-  const float sv = p.s_v, so = p.s_o;
-  const float hsv = 0.5f * sv, hso = 0.5f * so;
-  const float ux = m.ux, uy = m.uy, uz = m.uz;
-  const float c0 = m.drho - 1.5f * (ux * ux + uy * uy + uz * uz);
-  float K0 = 0.f, K1 = 0.f, Fa = 0.f;
-  const float Fx = p.fx, Fy = p.fy, Fz = p.fz;
+// rho and v of streaming3 (SP:380-388): v = (j + F/2) / rho.  `first` drops
+// the half-force shift: init's state carries v = u0 exactly (SP:165-166).
+template <typename R>
+__device__ __forceinline__ Macro<R> macros_pd(R g0, const R P[10], const R D[10], const Phys<R> &ph, bool first) {
+  Macro<R> m;
+  m.drho = g0 + ((P[1] + P[2]) + (P[3] + P[4])) + ((P[5] + P[6]) + (P[7] + P[8]) + P[9]);
+  // e_lo of the pairs: 1 (1,0,0) 2 (0,1,0) 3 (0,0,1) 4 (1,1,0) 5 (1,-1,0)
+  //                    6 (1,0,1) 7 (1,0,-1) 8 (0,1,1) 9 (0,1,-1)
+  m.jx = D[1] + (D[4] + D[5]) + (D[6] + D[7]);
+  m.jy = D[2] + (D[4] - D[5]) + (D[8] + D[9]);
+  m.jz = D[3] + (D[6] - D[7]) + (D[8] - D[9]);
+  const R inv = R(1) / (R(1) + m.drho);
+  if (first) {
+    m.ux = m.jx * inv;
+    m.uy = m.jy * inv;
+    m.uz = m.jz * inv;
+  } else {
+    m.ux = (m.jx + R(0.5) * ph.fx) * inv;
+    m.uy = (m.jy + R(0.5) * ph.fy) * inv;
+    m.uz = (m.jz + R(0.5) * ph.fz) * inv;
+  }
+  return m;
+}
+
+template <typename R, bool FORCED>
+__device__ __forceinline__ void collide_impl(R g[kQ], const R P[10], const R D[10], const Macro<R> &m,
+                                             const Phys<R> &ph) {
+  const R sv = ph.sv, so = ph.so;
+  const R hsv = R(0.5) * sv, hso = R(0.5) * so;
+  const R ux = m.ux, uy = m.uy, uz = m.uz;
+  const R c0 = m.drho - R(1.5) * (ux * ux + uy * uy + uz * uz);
+  R K0 = R(0), K1 = R(0), Fa = R(0);
+  const R Fx = ph.fx, Fy = ph.fy, Fz = ph.fz;
   if constexpr (FORCED) {
-    const float a = p.guo_a, b = p.guo_b;
-    const float uF = ux * Fx + uy * Fy + uz * Fz;
-    const float G0 = -(3.0f * a - b) * (1.0f / 3.0f) * uF;
-    K0 = -(1.0f - hsv) * a * uF + hsv * G0;
-    K1 = (1.0f - hsv) * b;
+    const R a = ph.ga, b = ph.gb;
+    const R uF = ux * Fx + uy * Fy + uz * Fz;
+    const R G0 = -(R(3) * a - b) * (R(1) / R(3)) * uF;
+    K0 = -(R(1) - hsv) * a * uF + hsv * G0;
+    K1 = (R(1) - hsv) * b;
     Fa = a;
   }
   // rest direction
   {
-    const float w = wq(0);
+    const R w = wq<R>(0);
     g[0] = g[0] - sv * (g[0] - w * c0);
     if constexpr (FORCED) g[0] += w * K0;
   }
 #pragma unroll
   for (int k = 1; k <= 9; ++k) {
     const int q = 2 * k - 1;
-    const float w = wq(q);
-    const float eu = edot(q, ux, uy, uz);
-    const float ej = edot(q, m.jx, m.jy, m.jz);
-    const float ap = hsv * P[k] - (sv * w) * (c0 + 4.5f * eu * eu);  // s_v (g+ - E+)
-    const float am = hso * D[k] - (so * 3.0f * w) * ej;              // s_o (g- - E-)
-    float lo = g[q] - ap - am, hi = g[q + 1] - ap + am;
+    const R w = wq<R>(q);
+    const R eu = edot<R>(q, ux, uy, uz);
+    const R ej = edot<R>(q, m.jx, m.jy, m.jz);
+    const R ap = hsv * P[k] - (sv * w) * (c0 + R(4.5) * eu * eu);  // s_v (g+ - E+)
+    const R am = hso * D[k] - (so * R(3) * w) * ej;                // s_o (g- - E-)
+    R lo = g[q] - ap - am, hi = g[q + 1] - ap + am;
     if constexpr (FORCED) {
-      const float eF = edot(q, Fx, Fy, Fz);
-      const float fp = w * (K0 + K1 * eu * eF);
-      const float fm = (w * Fa) * eF;
+      const R eF = edot<R>(q, Fx, Fy, Fz);
+      const R fp = w * (K0 + K1 * eu * eF);
+      const R fm = (w * Fa) * eF;
       lo += fp + fm;
       hi += fp - fm;
     }
@@ -432,9 +486,10 @@ __device__ __forceinline__ void collide_impl(float g[kQ], const float P[10], con
 }
 
 // feq(q, rho, u) - w_q  (SP:152-158, shifted)
-__device__ __forceinline__ float feq_shift(int q, float rho, float ux, float uy, float uz, float uu) {
-  const float eu = edot(q, ux, uy, uz);
-  return wq(q) * (rho - 1.0f) + wq(q) * rho * (3.0f * eu + 4.5f * eu * eu - 1.5f * uu);
+template <typename R>
+__device__ __forceinline__ R feq_shift(int q, R rho, R ux, R uy, R uz, R uu) {
+  const R eu = edot<R>(q, ux, uy, uz);
+  return wq<R>(q) * (rho - R(1)) + wq<R>(q) * rho * (R(3) * eu + R(4.5) * eu * eu - R(1.5) * uu);
 }
 
 // ---------------------------------------------------------------------------
@@ -465,30 +520,32 @@ __device__ __forceinline__ uint32_t plane_index(const lbm3d_params &p, int f, in
 // replaced by feq of the owner face.  Pressure (mode 1): rho_bc and the node's
 // previous-step velocity, or 0 when the inward neighbour is solid (its v is 0,
 // SP:390-392).  Velocity (mode 2): rho 1 and the face velocity.
-__device__ __forceinline__ void face_populations(float v[kQ], const KP &kp, int owner, bool inward_solid,
-                                                 const float *__restrict__ face_u, uint32_t pidx) {
+template <typename R>
+__device__ __forceinline__ void face_populations(R v[kQ], const KP &kp, int owner, bool inward_solid,
+                                                 const R *__restrict__ face_u, uint32_t pidx) {
   const lbm3d_params &p = kp.p;
-  float rho, ux, uy, uz;
+  const Phys<R> &ph = phys<R>(kp);
+  R rho, ux, uy, uz;
   if (p.bc_mode[owner] == 1) {
-    rho = p.rho_bc[owner];
+    rho = ph.rho_bc[owner];
     if (inward_solid) {
-      ux = uy = uz = 0.0f;
+      ux = uy = uz = R(0);
     } else {
-      const float *fu = face_u + p.face_u_off[owner];
+      const R *fu = face_u + p.face_u_off[owner];
       const uint32_t P = kp.plane[owner];
       ux = fu[pidx];
       uy = fu[P + pidx];
       uz = fu[2 * P + pidx];
     }
   } else {
-    rho = 1.0f;
-    ux = p.u_bc[owner][0];
-    uy = p.u_bc[owner][1];
-    uz = p.u_bc[owner][2];
+    rho = R(1);
+    ux = ph.u_bc[owner][0];
+    uy = ph.u_bc[owner][1];
+    uz = ph.u_bc[owner][2];
   }
-  const float uu = ux * ux + uy * uy + uz * uz;
+  const R uu = ux * ux + uy * uy + uz * uz;
 #pragma unroll
-  for (int q = 0; q < kQ; ++q) v[q] = feq_shift(q, rho, ux, uy, uz, uu);
+  for (int q = 0; q < kQ; ++q) v[q] = feq_shift<R>(q, rho, ux, uy, uz, uu);
 }
 
 // ---------------------------------------------------------------------------
@@ -511,7 +568,7 @@ struct CellCtx {
 
 template <typename T, bool ODD>
 __device__ __forceinline__ void read_cell(const Slots<T> &sl, const KP &kp, const int32_t *__restrict__ cells,
-                                          const Rows &rb, uint32_t x, float v[kQ], uint32_t off[kQ],
+                                          const Rows &rb, uint32_t x, typename St<T>::R v[kQ], uint32_t off[kQ],
                                           CellCtx &cc) {
   const uint32_t xb = x * (uint32_t)sizeof(T);
   if constexpr (!ODD) {
@@ -545,14 +602,16 @@ __device__ __forceinline__ uint32_t face_record(const uint32_t *__restrict__ fac
 template <typename T, bool ODD, bool FORCED>
 __global__ void __launch_bounds__(kBlock, LBM3D_STEP_MINB)
     k_step(const KP kp, T *__restrict__ g, const int32_t *__restrict__ cells, const uint4 *__restrict__ rows,
-           const uint32_t *__restrict__ face_bits, float *__restrict__ face_u, int first) {
+           const uint32_t *__restrict__ face_bits, typename St<T>::R *__restrict__ face_u, int first) {
+  using R = typename St<T>::R;
   const uint32_t x = logical_block<ODD ? LBM3D_XCD_REMAP_ODD : LBM3D_XCD_REMAP_EVEN,
                                    !ODD && LBM3D_REVERSE_EVEN>() * kBlock + threadIdx.x;
   const lbm3d_params &p = kp.p;
   if (x >= (uint32_t)p.n_cells) return;
+  const Phys<R> &ph = phys<R>(kp);
   const Slots<T> sl(g, kp.S);
   const Rows rb(rows, kp.n_row_words);
-  float v[kQ];
+  R v[kQ];
   uint32_t off[kQ];
   CellCtx cc;
   read_cell<T, ODD>(sl, kp, cells, rb, x, v, off, cc);
@@ -564,20 +623,20 @@ __global__ void __launch_bounds__(kBlock, LBM3D_STEP_MINB)
     if (!cc.have_coords) coords(kp, (uint32_t)cells[x], cc.i, cc.j, cc.k);
     owner = face_owner(p, cc.i, cc.j, cc.k);
     pidx = plane_index(p, owner, cc.i, cc.j, cc.k);
-    if (!first) face_populations(v, kp, owner, (fr & 2u) != 0, face_u, pidx);
+    if (!first) face_populations<R>(v, kp, owner, (fr & 2u) != 0, face_u, pidx);
   }
-  float P[10], D[10];
-  pair_sums(v, P, D);
-  const Macro m = macros_pd(v[0], P, D, p, first != 0);
+  R P[10], D[10];
+  pair_sums<R>(v, P, D);
+  const Macro<R> m = macros_pd<R>(v[0], P, D, ph, first != 0);
   if (owner >= 0 && p.bc_mode[owner] == 1 && !(fr & 2u)) {
     // the pressure face of the next step uses this step's v (SP:281)
-    float *fu = face_u + p.face_u_off[owner];
+    R *fu = face_u + p.face_u_off[owner];
     const uint32_t PL = kp.plane[owner];
     fu[pidx] = m.ux;
     fu[PL + pidx] = m.uy;
     fu[2 * PL + pidx] = m.uz;
   }
-  collide_impl<FORCED>(v, P, D, m, p);
+  collide_impl<R, FORCED>(v, P, D, m, ph);
 
   if constexpr (!ODD) {
     const uint32_t xb = x * (uint32_t)sizeof(T);
@@ -594,14 +653,16 @@ template <typename T, bool ODD>
 __global__ void __launch_bounds__(kBlock)
     k_macros(const KP kp, const T *__restrict__ g, const int32_t *__restrict__ cells,
              const uint4 *__restrict__ rows, const uint32_t *__restrict__ face_bits,
-             const float *__restrict__ face_u, int first, float *__restrict__ rho_out, float *__restrict__ v_out) {
+             const typename St<T>::R *__restrict__ face_u, int first, typename St<T>::R *__restrict__ rho_out,
+             typename St<T>::R *__restrict__ v_out) {
+  using R = typename St<T>::R;
   const uint32_t x = blockIdx.x * kBlock + threadIdx.x;
   const lbm3d_params &p = kp.p;
   const uint32_t N = (uint32_t)p.n_cells;
   if (x >= N) return;
   const Slots<T> sl(g, kp.S);
   const Rows rb(rows, kp.n_row_words);
-  float v[kQ];
+  R v[kQ];
   uint32_t off[kQ];
   CellCtx cc;
   read_cell<T, ODD>(sl, kp, cells, rb, x, v, off, cc);
@@ -609,12 +670,12 @@ __global__ void __launch_bounds__(kBlock)
   if ((fr & 1u) && !first) {
     if (!cc.have_coords) coords(kp, (uint32_t)cells[x], cc.i, cc.j, cc.k);
     const int owner = face_owner(p, cc.i, cc.j, cc.k);
-    face_populations(v, kp, owner, (fr & 2u) != 0, face_u, plane_index(p, owner, cc.i, cc.j, cc.k));
+    face_populations<R>(v, kp, owner, (fr & 2u) != 0, face_u, plane_index(p, owner, cc.i, cc.j, cc.k));
   }
-  float P[10], D[10];
-  pair_sums(v, P, D);
-  const Macro m = macros_pd(v[0], P, D, p, first != 0);
-  rho_out[x] = 1.0f + m.drho;
+  R P[10], D[10];
+  pair_sums<R>(v, P, D);
+  const Macro<R> m = macros_pd<R>(v[0], P, D, phys<R>(kp), first != 0);
+  rho_out[x] = R(1) + m.drho;
   v_out[x] = m.ux;
   v_out[(size_t)N + x] = m.uy;
   v_out[2 * (size_t)N + x] = m.uz;
@@ -622,29 +683,30 @@ __global__ void __launch_bounds__(kBlock)
 
 template <typename T>
 __global__ void __launch_bounds__(kBlock)
-    k_init(const KP kp, T *__restrict__ g, const int32_t *__restrict__ cells, float *__restrict__ face_u,
+    k_init(const KP kp, T *__restrict__ g, const int32_t *__restrict__ cells, typename St<T>::R *__restrict__ face_u,
            const float *__restrict__ rho0, const float *__restrict__ u0) {
+  using R = typename St<T>::R;
   const uint32_t x = blockIdx.x * kBlock + threadIdx.x;
   const lbm3d_params &p = kp.p;
   const uint32_t N = (uint32_t)p.n_cells;
   if (x >= N) return;
   const Slots<T> sl(g, kp.S);
-  float rho = 1.0f, ux = 0.f, uy = 0.f, uz = 0.f;
-  if (rho0) rho = rho0[x];
+  R rho = R(1), ux = R(0), uy = R(0), uz = R(0);
+  if (rho0) rho = (R)rho0[x];
   if (u0) {
-    ux = u0[x];
-    uy = u0[(size_t)N + x];
-    uz = u0[2 * (size_t)N + x];
+    ux = (R)u0[x];
+    uy = (R)u0[(size_t)N + x];
+    uz = (R)u0[2 * (size_t)N + x];
   }
-  const float uu = ux * ux + uy * uy + uz * uz;
+  const R uu = ux * ux + uy * uy + uz * uz;
   const uint32_t xb = x * (uint32_t)sizeof(T);
 #pragma unroll
-  for (int q = 0; q < kQ; ++q) sl.st_slot(q, xb, feq_shift(q, rho, ux, uy, uz, uu));
+  for (int q = 0; q < kQ; ++q) sl.st_slot(q, xb, feq_shift<R>(q, rho, ux, uy, uz, uu));
   int i, j, k;
   coords(kp, (uint32_t)cells[x], i, j, k);
   const int owner = face_owner(p, i, j, k);
   if (owner >= 0 && p.bc_mode[owner] == 1) {
-    float *fu = face_u + p.face_u_off[owner];
+    R *fu = face_u + p.face_u_off[owner];
     const uint32_t PL = kp.plane[owner], pi = plane_index(p, owner, i, j, k);
     fu[pi] = ux;
     fu[PL + pi] = uy;
@@ -910,6 +972,13 @@ int check_cells(const lbm3d_params *p) {
   return LBM3D_OK;
 }
 
+int check_dtype_size(const lbm3d_params *p, int dtype) {
+  const uint64_t elt = dtype == LBM3D_F64 ? 8 : (dtype == LBM3D_F32 ? 4 : 2);
+  if ((uint64_t)slot_stride((uint32_t)p->n_cells) * 2 * elt > UINT32_MAX)
+    return fail(LBM3D_ESIZE, "%s", "two slots of n_cells exceed a 32-bit buffer range for this dtype");
+  return LBM3D_OK;
+}
+
 int check_face_offsets(const lbm3d_params *p) {
   for (int f = 0; f < 6; ++f)
     if (p->bc_mode[f] == 1 && p->face_u_off[f] < 0)
@@ -917,7 +986,9 @@ int check_face_offsets(const lbm3d_params *p) {
   return LBM3D_OK;
 }
 
-bool valid_dtype(int d) { return d == LBM3D_F32 || d == LBM3D_F16 || d == LBM3D_BF16; }
+bool valid_dtype(int d) { return d == LBM3D_F32 || d == LBM3D_F16 || d == LBM3D_BF16 || d == LBM3D_F64; }
+
+int check_dtype_size(const lbm3d_params *p, int dtype);
 
 int solver_args(const lbm3d_params *p, int dtype, const void *g, const int32_t *cells, const void *rows,
                 const uint32_t *face_bits) {
@@ -926,6 +997,7 @@ int solver_args(const lbm3d_params *p, int dtype, const void *g, const int32_t *
   if ((rc = check_cells(p))) return rc;
   if ((rc = check_face_offsets(p))) return rc;
   if (!valid_dtype(dtype)) return fail(LBM3D_EINVAL, "%s", "unknown dtype");
+  if ((rc = check_dtype_size(p, dtype))) return rc;
   if (p->n_cells > 0 && (!g || !cells || !rows || !face_bits))
     return fail(LBM3D_EINVAL, "%s", "populations, cells, rows or face_bits is NULL");
   return LBM3D_OK;
@@ -933,17 +1005,18 @@ int solver_args(const lbm3d_params *p, int dtype, const void *g, const int32_t *
 
 template <typename T, bool FORCED>
 void launch_step_parity(hipStream_t s, const KP &kp, void *g, const int32_t *cells, const uint4 *rows,
-                        const uint32_t *face_bits, float *face_u, int64_t t) {
+                        const uint32_t *face_bits, void *face_u, int64_t t) {
+  using R = typename St<T>::R;
   const unsigned grid = grid_for(kp.p.n_cells);
   if (t & 1)
-    k_step<T, true, FORCED><<<grid, kBlock, 0, s>>>(kp, (T *)g, cells, rows, face_bits, face_u, t == 0);
+    k_step<T, true, FORCED><<<grid, kBlock, 0, s>>>(kp, (T *)g, cells, rows, face_bits, (R *)face_u, t == 0);
   else
-    k_step<T, false, FORCED><<<grid, kBlock, 0, s>>>(kp, (T *)g, cells, rows, face_bits, face_u, t == 0);
+    k_step<T, false, FORCED><<<grid, kBlock, 0, s>>>(kp, (T *)g, cells, rows, face_bits, (R *)face_u, t == 0);
 }
 
 template <typename T>
 int launch_step(hipStream_t s, const KP &kp, void *g, const int32_t *cells, const uint4 *rows,
-                const uint32_t *face_bits, float *face_u, int64_t t) {
+                const uint32_t *face_bits, void *face_u, int64_t t) {
   if (kp.p.n_cells == 0) return LBM3D_OK;
   if (kp.p.force_on)
     launch_step_parity<T, true>(s, kp, g, cells, rows, face_bits, face_u, t);
@@ -954,22 +1027,26 @@ int launch_step(hipStream_t s, const KP &kp, void *g, const int32_t *cells, cons
 
 template <typename T>
 int launch_macros(hipStream_t s, const KP &kp, const void *g, const int32_t *cells, const uint4 *rows,
-                  const uint32_t *face_bits, const float *face_u, int64_t t, float *rho, float *v) {
+                  const uint32_t *face_bits, const void *face_u, int64_t t, void *rho, void *v) {
+  using R = typename St<T>::R;
   const unsigned grid = (unsigned)((kp.p.n_cells + kBlock - 1) / kBlock);
   if (grid == 0) return LBM3D_OK;
   if (t & 1)
-    k_macros<T, true><<<grid, kBlock, 0, s>>>(kp, (const T *)g, cells, rows, face_bits, face_u, t == 0, rho, v);
+    k_macros<T, true><<<grid, kBlock, 0, s>>>(kp, (const T *)g, cells, rows, face_bits, (const R *)face_u, t == 0,
+                                              (R *)rho, (R *)v);
   else
-    k_macros<T, false><<<grid, kBlock, 0, s>>>(kp, (const T *)g, cells, rows, face_bits, face_u, t == 0, rho, v);
+    k_macros<T, false><<<grid, kBlock, 0, s>>>(kp, (const T *)g, cells, rows, face_bits, (const R *)face_u, t == 0,
+                                               (R *)rho, (R *)v);
   return check_launch("lbm3d_macros");
 }
 
 template <typename T>
-int launch_init(hipStream_t s, const KP &kp, void *g, const int32_t *cells, float *face_u, const float *rho0,
+int launch_init(hipStream_t s, const KP &kp, void *g, const int32_t *cells, void *face_u, const float *rho0,
                 const float *u0) {
+  using R = typename St<T>::R;
   const unsigned grid = (unsigned)((kp.p.n_cells + kBlock - 1) / kBlock);
   if (grid == 0) return LBM3D_OK;
-  k_init<T><<<grid, kBlock, 0, s>>>(kp, (T *)g, cells, face_u, rho0, u0);
+  k_init<T><<<grid, kBlock, 0, s>>>(kp, (T *)g, cells, (R *)face_u, rho0, u0);
   return check_launch("lbm3d_init");
 }
 
@@ -1071,13 +1148,14 @@ int lbm3d_build_neighbors(void *stream, const lbm3d_params *p, const int32_t *ce
   return check_launch("lbm3d_build_neighbors");
 }
 
-int lbm3d_init(void *stream, const lbm3d_params *p, int dtype, void *g, const int32_t *cells, float *face_u,
+int lbm3d_init(void *stream, const lbm3d_params *p, int dtype, void *g, const int32_t *cells, void *face_u,
                const float *rho0, const float *u0) {
   int rc = check_params(p);
   if (rc) return rc;
   if ((rc = check_cells(p))) return rc;
   if ((rc = check_face_offsets(p))) return rc;
   if (!valid_dtype(dtype)) return fail(LBM3D_EINVAL, "%s", "unknown dtype");
+  if ((rc = check_dtype_size(p, dtype))) return rc;
   if (p->n_cells > 0 && (!g || !cells)) return fail(LBM3D_EINVAL, "%s", "populations or cells are NULL");
   for (int f = 0; f < 6; ++f)
     if (p->bc_mode[f] == 1 && !face_u) return fail(LBM3D_EINVAL, "%s", "face_u is NULL with a pressure face");
@@ -1086,12 +1164,13 @@ int lbm3d_init(void *stream, const lbm3d_params *p, int dtype, void *g, const in
   switch (dtype) {
     case LBM3D_F32: return launch_init<float>(s, kp, g, cells, face_u, rho0, u0);
     case LBM3D_F16: return launch_init<__half>(s, kp, g, cells, face_u, rho0, u0);
+    case LBM3D_F64: return launch_init<double>(s, kp, g, cells, face_u, rho0, u0);
     default: return launch_init<__hip_bfloat16>(s, kp, g, cells, face_u, rho0, u0);
   }
 }
 
 int lbm3d_step(void *stream, const lbm3d_params *p, int dtype, void *g, const int32_t *cells,
-               const lbm3d_row_word *rows, const uint32_t *face_bits, float *face_u, int64_t t) {
+               const lbm3d_row_word *rows, const uint32_t *face_bits, void *face_u, int64_t t) {
   int rc = solver_args(p, dtype, g, cells, rows, face_bits);
   if (rc) return rc;
   if (t < 0) return fail(LBM3D_EINVAL, "%s", "negative step index");
@@ -1101,12 +1180,13 @@ int lbm3d_step(void *stream, const lbm3d_params *p, int dtype, void *g, const in
   switch (dtype) {
     case LBM3D_F32: return launch_step<float>(s, kp, g, cells, r, face_bits, face_u, t);
     case LBM3D_F16: return launch_step<__half>(s, kp, g, cells, r, face_bits, face_u, t);
+    case LBM3D_F64: return launch_step<double>(s, kp, g, cells, r, face_bits, face_u, t);
     default: return launch_step<__hip_bfloat16>(s, kp, g, cells, r, face_bits, face_u, t);
   }
 }
 
 int lbm3d_run(void *stream, const lbm3d_params *p, int dtype, void *g, const int32_t *cells,
-              const lbm3d_row_word *rows, const uint32_t *face_bits, float *face_u, int64_t t0, int32_t nsteps) {
+              const lbm3d_row_word *rows, const uint32_t *face_bits, void *face_u, int64_t t0, int32_t nsteps) {
   int rc = solver_args(p, dtype, g, cells, rows, face_bits);
   if (rc) return rc;
   if (t0 < 0 || nsteps < 0) return fail(LBM3D_EINVAL, "%s", "negative step index or count");
@@ -1117,6 +1197,7 @@ int lbm3d_run(void *stream, const lbm3d_params *p, int dtype, void *g, const int
     switch (dtype) {
       case LBM3D_F32: rc = launch_step<float>(s, kp, g, cells, r, face_bits, face_u, t0 + k); break;
       case LBM3D_F16: rc = launch_step<__half>(s, kp, g, cells, r, face_bits, face_u, t0 + k); break;
+      case LBM3D_F64: rc = launch_step<double>(s, kp, g, cells, r, face_bits, face_u, t0 + k); break;
       default: rc = launch_step<__hip_bfloat16>(s, kp, g, cells, r, face_bits, face_u, t0 + k); break;
     }
     if (rc) return rc;
@@ -1125,8 +1206,8 @@ int lbm3d_run(void *stream, const lbm3d_params *p, int dtype, void *g, const int
 }
 
 int lbm3d_macros(void *stream, const lbm3d_params *p, int dtype, const void *g, const int32_t *cells,
-                 const lbm3d_row_word *rows, const uint32_t *face_bits, const float *face_u, int64_t t, float *rho,
-                 float *v) {
+                 const lbm3d_row_word *rows, const uint32_t *face_bits, const void *face_u, int64_t t, void *rho,
+                 void *v) {
   int rc = solver_args(p, dtype, g, cells, rows, face_bits);
   if (rc) return rc;
   if (t < 0) return fail(LBM3D_EINVAL, "%s", "negative step index");
@@ -1137,6 +1218,7 @@ int lbm3d_macros(void *stream, const lbm3d_params *p, int dtype, const void *g, 
   switch (dtype) {
     case LBM3D_F32: return launch_macros<float>(s, kp, g, cells, r, face_bits, face_u, t, rho, v);
     case LBM3D_F16: return launch_macros<__half>(s, kp, g, cells, r, face_bits, face_u, t, rho, v);
+    case LBM3D_F64: return launch_macros<double>(s, kp, g, cells, r, face_bits, face_u, t, rho, v);
     default: return launch_macros<__hip_bfloat16>(s, kp, g, cells, r, face_bits, face_u, t, rho, v);
   }
 }

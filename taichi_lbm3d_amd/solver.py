@@ -115,8 +115,9 @@ class LB3D_Solver_Single_Phase:
     Extra keyword arguments (not in the reference):
       convention -- "reference" (SP's tau = niu/3+1/2 and /3, /9 force term;
                     the default), "documented" or "textbook";
-      dtype      -- population storage "f32" (default), "f16" or "bf16";
-                    stored shifted as f - w, arithmetic in FP32;
+      dtype      -- population storage "f32" (default), "f16" or "bf16",
+                    stored shifted as f - w with FP32 arithmetic; or "f64",
+                    FP64 storage and arithmetic (the FP64 GPU reference run);
       device     -- torch device index or string (default: current device).
     """
 
@@ -331,10 +332,15 @@ class LB3D_Solver_Single_Phase:
         torch = _torch()
         dev = self._torch_device()
         p = self._params
-        tdt = {"f32": torch.float32, "f16": torch.float16, "bf16": torch.bfloat16}[self.dtype]
+        tdt = {"f32": torch.float32, "f16": torch.float16, "bf16": torch.bfloat16, "f64": torch.float64}[self.dtype]
         stride = _lib.lib().size("lbm3d_slot_stride", p.n_cells)
         self._dev["g"] = torch.zeros(max(19 * stride, 1), dtype=tdt, device=dev)
-        self._dev["face_u"] = torch.zeros(max(self._face_u_floats, 1), dtype=torch.float32, device=dev)
+        self._dev["face_u"] = torch.zeros(max(self._face_u_floats, 1), dtype=self._real_dtype(), device=dev)
+
+    def _real_dtype(self):
+        """Arithmetic type of the kernels: FP64 for f64 storage, FP32 otherwise."""
+        import torch
+        return torch.float64 if self.dtype == "f64" else torch.float32
 
     def _init_state(self, rho0=None, u0=None):
         """init kernel (SP:160-170): equilibrium at (rho0, u0), default rest state."""
@@ -416,12 +422,36 @@ class LB3D_Solver_Single_Phase:
         return int(self._params.n_cells)
 
     # ---- macroscopic fields ---------------------------------------------------
-    def compact_macros(self):
-        """(rho [n], v [3,n]) torch tensors over the fluid list, after steps_done steps."""
+    def compact_macros(self, native: bool = False):
+        """(rho [n], v [3*n]) FP32 torch tensors over the fluid list, after steps_done
+        steps; with native=True in the kernels' arithmetic type (FP64 for f64)."""
+        if native and self.dtype == "f64":
+            return self._macros_native()
+        return self._macros_f32()
+
+    def _macros_native(self):
+        torch = _torch()
+        p = self._params
+        d = self._dev
+        dev = self._torch_device()
+        n = max(p.n_cells, 1)
+        rho = torch.empty(n, dtype=self._real_dtype(), device=dev)
+        v = torch.empty(3 * n, dtype=self._real_dtype(), device=dev)
+        with torch.cuda.device(dev):
+            _lib.lib().call("lbm3d_macros", self._stream(), _ref(p), _lib.DTYPES[self.dtype], _ptr(d["g"]),
+                            _ptr(d["cells"]), _ptr(d["rows"]), _ptr(d["face_bits"]), _ptr(d["face_u"]), self._t,
+                            _ptr(rho), _ptr(v))
+        return rho, v
+
+    def _macros_f32(self):
         if self._macro_cache is not None:
             return self._macro_cache
         if not self._initialised:
             raise RuntimeError("call init_simulation() first")
+        if self.dtype == "f64":
+            rho, v = self._macros_native()
+            self._macro_cache = (rho.float(), v.float())
+            return self._macro_cache
         torch = _torch()
         p = self._params
         d = self._dev

@@ -1,0 +1,181 @@
+"""GPU tests of the wider surface: storage types, the cavity configuration,
+warm start, checkpoints, export and full-size invariants."""
+import os
+
+import numpy as np
+import pytest
+
+from oracle import COracle
+from oracle import oracle_np as onp
+from test_gpu_parity import CASES, case_geometry, make_oracle, make_solver, rel_l2
+
+pytestmark = pytest.mark.gpu
+
+THREADS = min(16, len(os.sched_getaffinity(0)))
+
+
+@pytest.mark.parametrize("name", sorted(CASES))
+def test_f64_storage_matches_fp64_oracle(gpu, name):
+    """FP64 storage and arithmetic: the same algorithm as the FP64 oracle to round-off."""
+    solid = case_geometry(name)
+    args = CASES[name][2:]
+    s = make_solver(solid, *args, dtype="f64")
+    o = make_oracle(solid, *args)
+    s.run(31)
+    o.step(31)
+    fl = solid == 0
+    rho, v = s.compact_macros(native=True)
+    cells = s._dev["cells"].cpu().numpy()
+    n = s.n_fluid
+    rho = rho.cpu().numpy()[:n]
+    v = v.cpu().numpy()[: 3 * n].reshape(3, n).T
+    orho = o.rho().reshape(-1)[cells]
+    ov = o.v().reshape(-1, 3)[cells]
+    assert np.abs(rho - orho).max() < 1e-13
+    assert rel_l2(v, ov, floor=1e-12) < 1e-10
+    assert fl.sum() == n
+
+
+@pytest.mark.parametrize("dtype,tol", [("f16", 2e-2), ("bf16", 1e-1)])
+def test_half_storage_tracks_fp64(gpu, dtype, tol):
+    """Shifted FP16/BF16 storage with FP32 arithmetic stays close to FP64."""
+    name = "pressure_force"
+    solid = case_geometry(name)
+    args = CASES[name][2:]
+    s = make_solver(solid, *args, dtype=dtype)
+    o = make_oracle(solid, *args)
+    s.run(200)
+    o.step(200)
+    fl = solid == 0
+    rho, v = s.rho.to_numpy(), s.v.to_numpy()
+    assert np.isfinite(v).all()
+    ev = rel_l2(v[fl], o.v()[fl])
+    er = rel_l2(rho[fl], o.rho()[fl])
+    print(dtype, "rel-L2 v", ev, "rho", er)
+    assert ev < tol and er < 1e-3
+
+
+def cavity_setup(obj, n):
+    """Config: dense 64^3 all-fluid box, niu 0.1, velocity on all six faces,
+    top z face vx = 0.1 (the lid), others 0; rest-state start, no force."""
+    for face in ("x_left", "x_right", "y_left", "y_right", "z_left"):
+        obj(face, [0.0, 0.0, 0.0])
+    obj("z_right", [0.1, 0.0, 0.0])
+
+
+def test_lid_driven_cavity_64(gpu):
+    """Lid-driven cavity 64^3 x 2000 steps: rho and v within relative L2 1e-5 of
+    the FP64 restatement of the reference, and no further from it than the
+    reference's own FP32 arithmetic (the FP32 oracle) is."""
+    from taichi_lbm3d_amd import LB3D_Solver_Single_Phase
+    n, steps = 64, 2000
+    s = LB3D_Solver_Single_Phase(n, n, n)
+    s.niu = 0.1
+    setters = {"x_left": s.set_bc_vel_x0, "x_right": s.set_bc_vel_x1, "y_left": s.set_bc_vel_y0,
+               "y_right": s.set_bc_vel_y1, "z_left": s.set_bc_vel_z0, "z_right": s.set_bc_vel_z1}
+    cavity_setup(lambda f, u: setters[f](u), n)
+    s.init_simulation()
+    s.run(steps)
+    rho, v = s.rho.to_numpy(), s.v.to_numpy()
+    ref = {}
+    for real in ("f64", "f32"):
+        o = COracle(n, n, n, real=real, threads=THREADS)
+        o.niu = 0.1
+        cavity_setup(o.set_bc_vel, n)
+        o.init_simulation()
+        o.step(steps)
+        ref[real] = (o.rho(), o.v(), o.get_max_v())
+    er, ev = rel_l2(rho, ref["f64"][0]), rel_l2(v, ref["f64"][1])
+    er32, ev32 = rel_l2(ref["f32"][0], ref["f64"][0]), rel_l2(ref["f32"][1], ref["f64"][1])
+    print(f"cavity, GPU vs FP64 oracle: rel-L2 rho {er:.2e} v {ev:.2e}; "
+          f"FP32 oracle vs FP64 oracle: rho {er32:.2e} v {ev32:.2e}; max|v| {np.abs(v).max():.4f}")
+    assert er < 1e-5 and ev < 1e-5
+    assert ev <= ev32 and er <= er32
+    assert abs(s.get_max_v() - ref["f64"][2]) < 1e-5
+
+
+def test_checkpoint_resume_bit_exact(gpu, tmp_path):
+    """Stop after an odd step count, save, load into a fresh solver, continue:
+    identical to the uninterrupted run (the AA parity travels with the state)."""
+    name = "mixed_all_faces"
+    solid = case_geometry(name)
+    args = CASES[name][2:]
+    a = make_solver(solid, *args)
+    a.run(13)
+    path = str(tmp_path / "ck.pt")
+    a.save_checkpoint(path)
+    a.run(10)
+    b = make_solver(solid, *args)
+    b.load_checkpoint(path)
+    assert b.steps_done == 13
+    b.run(10)
+    np.testing.assert_array_equal(a.v.to_numpy(), b.v.to_numpy())
+    np.testing.assert_array_equal(a.rho.to_numpy(), b.rho.to_numpy())
+
+
+def test_warm_start_from_macros(gpu):
+    """init_from_macros(rho, v): f = feq(rho, v), and the first step collides with
+    exactly those macros, as the oracle does from the same state."""
+    name = "periodic_force"
+    solid = case_geometry(name)
+    args = CASES[name][2:]
+    rng = np.random.default_rng(11)
+    shape = solid.shape
+    rho0 = (1 + 1e-3 * rng.standard_normal(shape)).astype(np.float32)
+    u0 = (1e-3 * rng.standard_normal(shape + (3,))).astype(np.float32)
+    fl = solid == 0
+    s = make_solver(solid, *args)
+    s.init_from_macros(rho0, u0)
+    r0, v0 = s.rho.to_numpy(), s.v.to_numpy()
+    assert np.abs(r0[fl] - rho0[fl]).max() < 1e-6 and np.abs(v0[fl] - u0[fl]).max() < 1e-7
+    o = make_oracle(solid, *args)
+    rho_o = np.where(fl, rho0, 1.0).astype(np.float64)
+    u_o = np.where(fl[..., None], u0, 0.0).astype(np.float64)
+    F = onp.feq(rho_o, u_o)
+    F[~fl] = onp.W
+    o.set_F(F)
+    o.set_macros(rho_o, u_o)
+    s.run(5)
+    o.step(5)
+    assert rel_l2(s.v.to_numpy()[fl], o.v()[fl]) < 1e-5
+    assert rel_l2(s.rho.to_numpy()[fl], o.rho()[fl]) < 1e-6
+
+
+def test_export_vtk_roundtrip(gpu, tmp_path, monkeypatch):
+    from taichi_lbm3d_amd.vtk import read_vtr
+    name = "pressure_x"
+    solid = case_geometry(name)
+    s = make_solver(solid, *CASES[name][2:])
+    s.run(5)
+    monkeypatch.chdir(tmp_path)
+    fn = s.export_VTK(5)
+    assert os.path.basename(fn) == "LB_SingelPhase_5.vtr"
+    back = read_vtr(fn)
+    np.testing.assert_array_equal(back["rho"], s.rho.to_numpy())
+    np.testing.assert_array_equal(back["velocity"], s.v.to_numpy())
+    np.testing.assert_array_equal(back["Solid"], solid)
+
+
+def test_full_size_pack_invariants(gpu):
+    """256^3 sphere pack, all faces periodic, no force: total mass conserved to
+    FP32 round-off over 400 steps (both AA parities), fields finite, and the
+    neighbour ids bit-exact against the brute-force table."""
+    from taichi_lbm3d_amd import LB3D_Solver_Single_Phase
+    from taichi_lbm3d_amd.geometry import sphere_pack
+    from oracle import compaction as oc
+    solid = sphere_pack((256,) * 3, porosity=0.35, rmin=6.0, rmax=14.0, seed=0, periodic=(False, True, True))
+    s = LB3D_Solver_Single_Phase(256, 256, 256, sparse_storage=True)
+    s.solid.from_numpy(solid)
+    s.niu = 0.1
+    s.init_simulation()
+    cells, d2c = oc.compact(solid)
+    np.testing.assert_array_equal(s._dev["cells"].cpu().numpy(), cells)
+    np.testing.assert_array_equal(s.neighbor_table(), oc.neighbors(solid, cells, d2c))
+    rng = np.random.default_rng(1)
+    rho0 = (1 + 1e-3 * rng.standard_normal(solid.shape)).astype(np.float32)
+    s.init_from_macros(rho0, np.zeros(solid.shape + (3,), np.float32))
+    m0 = s.reduce()["sum_rho"]
+    s.run(401)
+    r = s.reduce()
+    assert r["nonfinite"] == 0
+    assert abs(r["sum_rho"] - m0) / m0 < 1e-6
