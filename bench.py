@@ -7,7 +7,7 @@ fixed pressure on the x faces (rho 1.0005 left, 1.0 right), y/z periodic,
 niu = 0.1, FP32 populations.  A step is one fused collide+stream launch over
 the fluid list.  Prints one JSON line.
 
-  python bench.py [--gpus 1] [--steps K] [--warmup W] [--dtype f32|f16|bf16]
+  python bench.py [--gpus 1] [--steps K] [--warmup W] [--dtype f32|f16|bf16|f64]
                   [--no-steady] [--no-cpu-baseline]
 """
 from __future__ import annotations
@@ -24,7 +24,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0           # MI355X HBM3E, GB/s
-BYTES_PER_FLUP = {"f32": 152, "f16": 76, "bf16": 76}  # 19 reads + 19 writes per fluid-cell update
+BYTES_PER_FLUP = {"f64": 304, "f32": 152, "f16": 76, "bf16": 76}  # 19 reads + 19 writes per fluid-cell update
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "step_traffic_pack256.json")
 
 
@@ -96,7 +96,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=200)
     ap.add_argument("--n", type=int, default=256)
     ap.add_argument("--seed", type=int, default=0)
-    ap.add_argument("--dtype", default="f32", choices=["f32", "f16", "bf16"])
+    ap.add_argument("--dtype", default="f32", choices=["f32", "f16", "bf16", "f64"])
     ap.add_argument("--no-steady", action="store_true")
     ap.add_argument("--max-steady-steps", type=int, default=50000)
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -165,6 +165,9 @@ def main():
     cpu = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(solid)
+        if steady is not None:
+            # CPU time to the same steady state, extrapolated from its measured step rate
+            cpu["steady_s_extrapolated"] = round(steady["steps"] * n_fluid / (cpu["value"] * 1e6), 1)
         log(f"cpu baseline: {cpu['value']:.1f} MLUPS on {cpu['cores']} cores")
 
     out = {

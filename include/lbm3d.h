@@ -39,7 +39,7 @@
 extern "C" {
 #endif
 
-#define LBM3D_ABI_VERSION 5
+#define LBM3D_ABI_VERSION 6
 
 /* storage types for populations; F32, F16 and BF16 compute in FP32, F64 in FP64 */
 #define LBM3D_F32 0
@@ -127,6 +127,10 @@ int lbm3d_compact_fill(void *stream, const lbm3d_params *p, const int8_t *solid,
 /* Row table (see lbm3d_row_word) from solid and the d2c map of pass 2. */
 int lbm3d_build_rows(void *stream, const lbm3d_params *p, const int8_t *solid, const int32_t *d2c,
                      lbm3d_row_word *rows);
+/* The same with every compact id (base, left, right) shifted by id_offset:
+ * the row table of one sample of a batch, whose cells start at id_offset. */
+int lbm3d_build_rows_at(void *stream, const lbm3d_params *p, const int8_t *solid, const int32_t *d2c,
+                        int32_t id_offset, lbm3d_row_word *rows);
 
 /* face_bits: per fluid cell, bit 2*(x%16) of word x/16 set when the cell
  * belongs to a face in mode 1 or 2 (owner = last such face in face order),
@@ -176,10 +180,11 @@ int lbm3d_macros(void *stream, const lbm3d_params *p, int dtype, const void *g, 
 
 /* Scatter compact rho/v into dense C-order arrays of extent (ox,oy,oz) >=
  * (nx,ny,nz): rho_dense[ox*oy*oz], v_dense[ox*oy*oz][3] (the to_numpy layout).
- * Cells that are not fluid are left untouched. */
+ * v is SoA with component c of cell x at v[c * v_stride + x] (v_stride 0 means
+ * n_cells).  Cells that are not fluid are left untouched. */
 int lbm3d_scatter_dense(void *stream, const lbm3d_params *p, const int32_t *cells,
-                        const float *rho, const float *v, int32_t ox, int32_t oy, int32_t oz,
-                        float *rho_dense, float *v_dense);
+                        const float *rho, const float *v, int64_t v_stride, int32_t ox, int32_t oy,
+                        int32_t oz, float *rho_dense, float *v_dense);
 
 /* FP64 reductions over the compact fields (cal_max_v, SP:399-402, plus the
  * flux sums used for permeability and convergence):
@@ -191,6 +196,101 @@ size_t lbm3d_reduce_workspace_bytes(int32_t n_cells);
 int lbm3d_reduce(void *stream, const lbm3d_params *p, const int32_t *cells, const float *rho,
                  const float *v, int32_t i_lo, int32_t i_hi, double *out, void *workspace,
                  size_t workspace_bytes);
+
+
+/* ---- batches: many samples of uneven size in one launch per step ----------
+ *
+ * No reference counterpart: the reference runs one grid per solver object
+ * and its drivers loop over samples in Python (example_porous_medium.py).
+ * Samples of a batch share the physical parameters and face modes of `p`
+ * (s_v, s_o, force, bc_mode, rho_bc, u_bc); each has its own grid.  Their
+ * fluid lists are laid end to end, sample s owning compact ids
+ * [cell_begin, cell_begin + n_cells) with cell_begin a multiple of
+ * LBM3D_BATCH_ALIGN, so the batch is one population array g[19][S] with
+ * S = lbm3d_slot_stride(cells) and one cells / face_bits / rows array:
+ *   cells[cell_begin + i]    dense C-order index inside sample s's own grid;
+ *   rows[rows_begin + ...]   sample s's row table, built with
+ *                            lbm3d_build_rows_at(..., id_offset = cell_begin);
+ *   face_bits                lbm3d_build_face_bits of sample s written at
+ *                            face_bits + cell_begin / 16;
+ *   face_u[face_u_begin ...] sample s's pressure-face state, faces in face order.
+ * In every batch call p->n_cells is the layout's total `cells` and p->nx,
+ * ny, nz are not used.  Padding ids between samples are never read or written.
+ */
+#define LBM3D_BATCH_ALIGN 512
+
+typedef struct lbm3d_sample {
+  int32_t nx, ny, nz;      /* the sample's grid */
+  int32_t n_cells;         /* its fluid cells (lbm3d_compact_count) */
+  int32_t cell_begin;      /* set by lbm3d_batch_layout */
+  int32_t face_u_begin;    /* set by lbm3d_batch_layout */
+  int64_t rows_begin;      /* set by lbm3d_batch_layout */
+} lbm3d_sample;
+
+typedef struct lbm3d_batch_sizes {
+  int64_t cells;       /* compact ids spanned (a multiple of LBM3D_BATCH_ALIGN): p->n_cells of batch calls */
+  int64_t row_words;   /* entries of the batch row table */
+  int64_t face_u;      /* elements of face_u */
+  int64_t desc_bytes;  /* bytes of the descriptor buffer */
+} lbm3d_batch_sizes;
+
+/* Launch shape of one parity of the batched step: `block` threads per
+ * workgroup (128, 256 or 512); grid_per_cu 0 = one workgroup per tile of
+ * `block` cells, otherwise a persistent grid of grid_per_cu workgroups per
+ * compute unit walking the tiles; reverse 1 = tiles in descending order. */
+typedef struct lbm3d_launch {
+  int32_t block;
+  int32_t grid_per_cu;
+  int32_t reverse;
+  int32_t reserved;
+} lbm3d_launch;
+
+/* Place n_samples samples (nx, ny, nz, n_cells filled in by the caller): sets
+ * cell_begin, face_u_begin, rows_begin of each and the batch sizes.  Checks
+ * each sample's extents against the face modes of p (host only). */
+int lbm3d_batch_layout(const lbm3d_params *p, lbm3d_sample *samples, int32_t n_samples,
+                       lbm3d_batch_sizes *sizes);
+
+/* Fill the descriptor (per-sample geometry and the chunk -> sample map) in
+ * host memory `host_desc` of sizes.desc_bytes; the caller copies it to a
+ * device buffer `desc` that the batch calls read (host only). */
+int lbm3d_batch_desc(const lbm3d_params *p, const lbm3d_sample *samples, int32_t n_samples,
+                     void *host_desc, size_t bytes);
+
+/* lbm3d_init over every sample of the batch; rho0 / u0 (SoA [3][cells]) in
+ * batch ids, or NULL. */
+int lbm3d_batch_init(void *stream, const lbm3d_params *p, int dtype, void *g, const void *desc,
+                     int32_t n_samples, const int32_t *cells, void *face_u, const float *rho0,
+                     const float *u0);
+
+/* One time step of every active sample (active[s] != 0, or all when active
+ * is NULL) in one launch; cfg[0] / cfg[1] the launch shapes of even / odd
+ * steps, or NULL for the defaults.  A sample's results do not depend on cfg,
+ * on the other samples or on the batch: they are bit-identical to lbm3d_step
+ * on that sample alone. */
+int lbm3d_batch_step(void *stream, const lbm3d_params *p, int dtype, void *g, const void *desc,
+                     int32_t n_samples, const int32_t *cells, const lbm3d_row_word *rows,
+                     const uint32_t *face_bits, void *face_u, const uint8_t *active, int64_t t,
+                     const lbm3d_launch *cfg);
+
+/* `nsteps` consecutive lbm3d_batch_step calls from step t0. */
+int lbm3d_batch_run(void *stream, const lbm3d_params *p, int dtype, void *g, const void *desc,
+                    int32_t n_samples, const int32_t *cells, const lbm3d_row_word *rows,
+                    const uint32_t *face_bits, void *face_u, const uint8_t *active, int64_t t0,
+                    int32_t nsteps, const lbm3d_launch *cfg);
+
+/* lbm3d_macros over the batch: rho[cells], v SoA [3][cells] in batch ids. */
+int lbm3d_batch_macros(void *stream, const lbm3d_params *p, int dtype, const void *g, const void *desc,
+                       int32_t n_samples, const int32_t *cells, const lbm3d_row_word *rows,
+                       const uint32_t *face_bits, const void *face_u, int64_t t, void *rho, void *v);
+
+/* Per-sample FP64 reductions of FP32 batch macros: out (device double
+ * [n_samples][8]) = sum rho, sum v (3), max |v|, non-finite cells, fluid
+ * cells, 0. */
+size_t lbm3d_batch_reduce_workspace_bytes(int64_t cells);
+int lbm3d_batch_reduce(void *stream, const lbm3d_params *p, const void *desc, int32_t n_samples,
+                       const float *rho, const float *v, double *out, void *workspace,
+                       size_t workspace_bytes);
 
 #ifdef __cplusplus
 }

@@ -1,0 +1,67 @@
+/*
+ * lbm3d_conv.h -- C ABI of the 3-D convolution kernels of liblbm3d.so: BF16
+ * implicit-GEMM convolutions on the gfx950 matrix cores (MFMA) with FP32
+ * accumulation, forward and backward, for the warm-start surrogate.
+ *
+ * No reference counterpart: the reference initialises every run at rest
+ * (init, Single_phase/LBM_3D_SinglePhase_Solver.py:160-170).  The surrogate
+ * predicts steady fields from the geometry and lbm3d_init starts the solver
+ * at their equilibrium instead.
+ *
+ * Layouts (all device pointers, caller-owned; nothing is allocated):
+ *   activations  BF16, channels last: x[((b*D + z)*H + y)*W + x][c], for a batch
+ *                of `n` volumes of D x H x W voxels and C channels;
+ *   weights      BF16 w[co][k], k = tap*cin + ci, tap = (dz+1)*9 + (dy+1)*3 +
+ *                (dx+1) for ks = 3 (tap 0 for ks = 1), rows padded with zeros
+ *                to kpad = lbm3d_conv_kpad(cin, ks) (a multiple of 32);
+ *   bias         FP32 b[cout] (may be NULL);
+ *   weight grads FP32 dw[co][kpad].
+ * Convolutions are "same" (stride 1, zero padding ks/2).  cin and cout must
+ * be multiples of 16, at most 256.  Return 0 or a negative LBM3D_E* code
+ * (lbm3d.h); lbm3d_last_error() has no entry for these, lbm3d_conv_last_error() does.
+ */
+#ifndef LBM3D_CONV_H
+#define LBM3D_CONV_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+const char *lbm3d_conv_last_error(void);
+
+/* Padded row length of the weight matrix: ks^3 * cin rounded up to 32. */
+int32_t lbm3d_conv_kpad(int32_t cin, int32_t ks);
+
+/* y = act(conv(x, w) + bias), act = ReLU when relu != 0, BF16 out (FP32 accumulation). */
+int lbm3d_conv3d_fwd(void *stream, const uint16_t *x, const uint16_t *w, const float *bias, uint16_t *y,
+                     int32_t n, int32_t d, int32_t h, int32_t wd, int32_t cin, int32_t cout, int32_t ks,
+                     int32_t relu);
+
+/* Weights of the data-gradient convolution: wt[ci][tap'*cout + co] =
+ * w[co][tap*cin + ci] with tap' = ks^3 - 1 - tap (the kernel flipped and
+ * its channel roles swapped), rows padded to lbm3d_conv_kpad(cout, ks). */
+int lbm3d_conv3d_wflip(void *stream, const uint16_t *w, uint16_t *wt, int32_t cin, int32_t cout, int32_t ks);
+
+/* dx = conv(dy, wt): the gradient of a convolution's input, given its output
+ * gradient dy (BF16, cout channels) and the flipped weights of lbm3d_conv3d_wflip. */
+int lbm3d_conv3d_dgrad(void *stream, const uint16_t *dy, const uint16_t *wt, uint16_t *dx, int32_t n,
+                       int32_t d, int32_t h, int32_t wd, int32_t cin, int32_t cout, int32_t ks);
+
+/* dw[co][tap*cin + ci] = sum over voxels of dy[v][co] * x[v + off(tap)][ci]
+ * (FP32; the padding columns are set to 0), through per-block partial sums in
+ * `workspace` of lbm3d_conv3d_wgrad_workspace_bytes() bytes, folded in a fixed
+ * order (deterministic). */
+size_t lbm3d_conv3d_wgrad_workspace_bytes(int32_t n, int32_t d, int32_t h, int32_t wd, int32_t cin,
+                                          int32_t cout, int32_t ks);
+int lbm3d_conv3d_wgrad(void *stream, const uint16_t *x, const uint16_t *dy, float *dw, int32_t n, int32_t d,
+                       int32_t h, int32_t wd, int32_t cin, int32_t cout, int32_t ks, void *workspace,
+                       size_t workspace_bytes);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* LBM3D_CONV_H */

@@ -12,16 +12,19 @@ import subprocess
 _HERE = os.path.dirname(os.path.abspath(__file__))
 _ROOT = os.path.dirname(_HERE)
 LIB_PATH = os.path.join(_HERE, "liblbm3d.so")
-SOURCES = [os.path.join(_HERE, "csrc", "lbm3d.hip")]
+SOURCES = [os.path.join(_HERE, "csrc", "lbm3d.hip"), os.path.join(_HERE, "csrc", "conv3d.hip")]
 INCLUDE = os.path.join(_ROOT, "include")
 
-ABI_VERSION = 5
+ABI_VERSION = 6
 F32, F16, BF16, F64 = 0, 1, 2, 3
 DTYPES = {"f32": F32, "f16": F16, "bf16": BF16, "f64": F64}
 HIPCC_FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",
     # keep FP32 arithmetic unpacked (no v_pk_*_f32)
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    # fuse a*b+c into an FMA only inside one source expression: the per-cell code
+    # then rounds identically in every kernel that inlines it (single grid, batch)
+    "-ffp-contract=on",
 ]
 
 
@@ -55,6 +58,29 @@ class Params(ctypes.Structure):
     ]
 
 
+class Sample(ctypes.Structure):
+    """Mirror of struct lbm3d_sample."""
+    _fields_ = [
+        ("nx", ctypes.c_int32), ("ny", ctypes.c_int32), ("nz", ctypes.c_int32),
+        ("n_cells", ctypes.c_int32), ("cell_begin", ctypes.c_int32), ("face_u_begin", ctypes.c_int32),
+        ("rows_begin", ctypes.c_int64),
+    ]
+
+
+class BatchSizes(ctypes.Structure):
+    """Mirror of struct lbm3d_batch_sizes."""
+    _fields_ = [("cells", ctypes.c_int64), ("row_words", ctypes.c_int64), ("face_u", ctypes.c_int64),
+                ("desc_bytes", ctypes.c_int64)]
+
+
+class Launch(ctypes.Structure):
+    """Mirror of struct lbm3d_launch."""
+    _fields_ = [("block", ctypes.c_int32), ("grid_per_cu", ctypes.c_int32), ("reverse", ctypes.c_int32),
+                ("reserved", ctypes.c_int32)]
+
+
+BATCH_ALIGN = 512  # LBM3D_BATCH_ALIGN
+
 # name -> (restype, argtypes); every symbol include/lbm3d.h declares
 _vp, _i32, _i64, _sz, _int = ctypes.c_void_p, ctypes.c_int32, ctypes.c_int64, ctypes.c_size_t, ctypes.c_int
 _pp = ctypes.POINTER(Params)
@@ -69,15 +95,34 @@ SIGNATURES = {
     "lbm3d_compact_count": (_int, [_vp, _pp, _vp, _vp, _vp, _sz]),
     "lbm3d_compact_fill": (_int, [_vp, _pp, _vp, _vp, _vp, _vp, _sz]),
     "lbm3d_build_rows": (_int, [_vp, _pp, _vp, _vp, _vp]),
+    "lbm3d_build_rows_at": (_int, [_vp, _pp, _vp, _vp, _i32, _vp]),
     "lbm3d_build_face_bits": (_int, [_vp, _pp, _vp, _vp, _vp]),
     "lbm3d_build_neighbors": (_int, [_vp, _pp, _vp, _vp, _vp]),
     "lbm3d_init": (_int, [_vp, _pp, _int, _vp, _vp, _vp, _vp, _vp]),
     "lbm3d_step": (_int, [_vp, _pp, _int, _vp, _vp, _vp, _vp, _vp, _i64]),
     "lbm3d_run": (_int, [_vp, _pp, _int, _vp, _vp, _vp, _vp, _vp, _i64, _i32]),
     "lbm3d_macros": (_int, [_vp, _pp, _int, _vp, _vp, _vp, _vp, _vp, _i64, _vp, _vp]),
-    "lbm3d_scatter_dense": (_int, [_vp, _pp, _vp, _vp, _vp, _i32, _i32, _i32, _vp, _vp]),
+    "lbm3d_scatter_dense": (_int, [_vp, _pp, _vp, _vp, _vp, _i64, _i32, _i32, _i32, _vp, _vp]),
     "lbm3d_reduce_workspace_bytes": (_sz, [_i32]),
     "lbm3d_reduce": (_int, [_vp, _pp, _vp, _vp, _vp, _i32, _i32, _vp, _vp, _sz]),
+    "lbm3d_batch_layout": (_int, [_pp, ctypes.POINTER(Sample), _i32, ctypes.POINTER(BatchSizes)]),
+    "lbm3d_batch_desc": (_int, [_pp, ctypes.POINTER(Sample), _i32, _vp, _sz]),
+    "lbm3d_batch_init": (_int, [_vp, _pp, _int, _vp, _vp, _i32, _vp, _vp, _vp, _vp]),
+    "lbm3d_batch_step": (_int, [_vp, _pp, _int, _vp, _vp, _i32, _vp, _vp, _vp, _vp, _vp, _i64,
+                                ctypes.POINTER(Launch)]),
+    "lbm3d_batch_run": (_int, [_vp, _pp, _int, _vp, _vp, _i32, _vp, _vp, _vp, _vp, _vp, _i64, _i32,
+                               ctypes.POINTER(Launch)]),
+    "lbm3d_batch_macros": (_int, [_vp, _pp, _int, _vp, _vp, _i32, _vp, _vp, _vp, _vp, _i64, _vp, _vp]),
+    "lbm3d_batch_reduce_workspace_bytes": (_sz, [_i64]),
+    "lbm3d_batch_reduce": (_int, [_vp, _pp, _vp, _i32, _vp, _vp, _vp, _vp, _sz]),
+    # include/lbm3d_conv.h
+    "lbm3d_conv_last_error": (ctypes.c_char_p, []),
+    "lbm3d_conv_kpad": (_i32, [_i32, _i32]),
+    "lbm3d_conv3d_fwd": (_int, [_vp, _vp, _vp, _vp, _vp, _i32, _i32, _i32, _i32, _i32, _i32, _i32, _i32]),
+    "lbm3d_conv3d_wflip": (_int, [_vp, _vp, _vp, _i32, _i32, _i32]),
+    "lbm3d_conv3d_dgrad": (_int, [_vp, _vp, _vp, _vp, _i32, _i32, _i32, _i32, _i32, _i32, _i32]),
+    "lbm3d_conv3d_wgrad_workspace_bytes": (_sz, [_i32, _i32, _i32, _i32, _i32, _i32, _i32]),
+    "lbm3d_conv3d_wgrad": (_int, [_vp, _vp, _vp, _vp, _i32, _i32, _i32, _i32, _i32, _i32, _i32, _vp, _sz]),
 }
 
 
@@ -110,7 +155,9 @@ class Lib:
     def call(self, name: str, *args):
         rc = getattr(self.cdll, name)(*args)
         if rc != 0:
-            raise LbmError(f"{name} failed ({rc}): {self.last_error()}")
+            msg = self.cdll.lbm3d_conv_last_error() if name.startswith("lbm3d_conv") else None
+            detail = msg.decode() if msg else self.last_error()
+            raise LbmError(f"{name} failed ({rc}): {detail}")
         return rc
 
     def size(self, name: str, *args) -> int:

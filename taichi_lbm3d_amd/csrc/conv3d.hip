@@ -1,0 +1,405 @@
+// 3-D "same" convolutions for the warm-start surrogate: BF16 implicit GEMM on
+// the gfx950 matrix cores (v_mfma_f32_16x16x32_bf16), FP32 accumulation.  The C
+// ABI is declared in include/lbm3d_conv.h.
+//
+// Forward (and data gradient, which is a forward convolution of the output
+// gradient with flipped weights): GEMM with M = voxels, N = output channels,
+// K = taps x input channels.  Activations are channels-last, so the 8
+// consecutive K values a lane feeds to one MFMA are 8 consecutive channels of
+// one (shifted) voxel: a single 16-byte load.  A workgroup of 4 waves computes
+// 64 voxels x 16*NT channels; each wave owns 16 voxels.
+//
+// Weight gradient: GEMM with M = output channels, N = input channels, K =
+// voxels.  Channels-last data must be transposed for that K, so each wave
+// stages 32 voxels of dy and of the tap-shifted x in LDS as [channel][voxel]
+// rows and reads its MFMA operands from there.  Workgroups own (voxel block,
+// tap, 16 output channels) and write partial sums; a second kernel folds them
+// in a fixed order.
+#include <hip/hip_runtime.h>
+#include <hip/hip_bf16.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <cstring>
+
+#include "lbm3d.h"
+#include "lbm3d_conv.h"
+
+namespace {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+
+thread_local char c_err[256] = "";
+
+int cfail(int code, const char *msg) {
+  snprintf(c_err, sizeof(c_err), "%s", msg);
+  return code;
+}
+
+int ccheck(const char *what) {
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) {
+    snprintf(c_err, sizeof(c_err), "%s: %s", what, hipGetErrorString(e));
+    return LBM3D_EHIP;
+  }
+  return LBM3D_OK;
+}
+
+// exact c / d for c < 2^31: (c * m) >> s
+struct Div {
+  uint32_t d, m, s;
+};
+Div make_div(uint32_t d) {
+  uint32_t l = 0;
+  while ((1ull << l) < d) ++l;
+  Div r;
+  r.d = d;
+  r.m = (uint32_t)(((1ull << (31 + l)) + d - 1) / d);
+  r.s = 31 + l;
+  return r;
+}
+__device__ __forceinline__ uint32_t qdiv(uint32_t c, const Div &v) { return (uint32_t)(((uint64_t)c * v.m) >> v.s); }
+
+struct Vol {
+  int32_t n, d, h, w;
+  Div dw, dh, dd;
+  uint32_t vox;  // n*d*h*w
+};
+
+Vol make_vol(int32_t n, int32_t d, int32_t h, int32_t w) {
+  Vol v;
+  v.n = n;
+  v.d = d;
+  v.h = h;
+  v.w = w;
+  v.dw = make_div((uint32_t)w);
+  v.dh = make_div((uint32_t)h);
+  v.dd = make_div((uint32_t)d);
+  v.vox = (uint32_t)((int64_t)n * d * h * w);
+  return v;
+}
+
+// voxel index -> (b, z, y, x)
+__device__ __forceinline__ void decode(const Vol &v, uint32_t m, int &b, int &z, int &y, int &x) {
+  const uint32_t t = qdiv(m, v.dw);
+  x = (int)(m - t * (uint32_t)v.w);
+  const uint32_t t2 = qdiv(t, v.dh);
+  y = (int)(t - t2 * (uint32_t)v.h);
+  const uint32_t bb = qdiv(t2, v.dd);
+  z = (int)(t2 - bb * (uint32_t)v.d);
+  b = (int)bb;
+}
+
+// tap -> offset (dz, dy, dx); ks = 1 has the single tap 0 at offset 0
+__device__ __forceinline__ void tap_offset(int tap, int ks, int &dz, int &dy, int &dx) {
+  if (ks == 3) {
+    dz = tap / 9 - 1;
+    dy = (tap / 3) % 3 - 1;
+    dx = tap % 3 - 1;
+  } else {
+    dz = dy = dx = 0;
+  }
+}
+
+// neighbour voxel of (b,z,y,x) at tap offset, or -1 outside the volume (zero padding)
+__device__ __forceinline__ int64_t shifted(const Vol &v, int b, int z, int y, int x, int dz, int dy, int dx) {
+  const int z2 = z + dz, y2 = y + dy, x2 = x + dx;
+  if (z2 < 0 || z2 >= v.d || y2 < 0 || y2 >= v.h || x2 < 0 || x2 >= v.w) return -1;
+  return (((int64_t)b * v.d + z2) * v.h + y2) * v.w + x2;
+}
+
+__device__ __forceinline__ u32x4 ld16(const uint16_t *p) { return *reinterpret_cast<const u32x4 *>(p); }
+
+__device__ __forceinline__ uint16_t to_bf16(float f) {
+  const __hip_bfloat16 h = __float2bfloat16(f);  // round to nearest even
+  return *reinterpret_cast<const uint16_t *>(&h);
+}
+
+// ---------------------------------------------------------------------------
+// Forward: y[v][co] = act(sum_{tap, ci} x[v + off(tap)][ci] * w[co][tap*cin + ci] + bias[co])
+// MFMA 16x16x32 lane layout: A (16 x 32): lane l holds row l%16, k = 8*(l/16) .. +7;
+// B (32 x 16): column l%16, the same k; C (16 x 16): column l%16, rows 4*(l/16) .. +3.
+// ---------------------------------------------------------------------------
+template <int NT>
+__global__ void __launch_bounds__(256)
+    k_conv_fwd(const uint16_t *__restrict__ x, const uint16_t *__restrict__ w, const float *__restrict__ bias,
+               uint16_t *__restrict__ y, const Vol vol, int cin, int cout, int ks, int kpad, int relu) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int g = lane >> 4, r16 = lane & 15;
+  const uint32_t m0 = blockIdx.x * 64u + (uint32_t)wave * 16u;
+  const int co0 = blockIdx.y * 16 * NT;
+  const uint32_t m = m0 + (uint32_t)r16;
+  const bool mvalid = m < vol.vox;
+  int vb = 0, vz = 0, vy = 0, vx = 0;
+  if (mvalid) decode(vol, m, vb, vz, vy, vx);
+  const int K = ks * ks * ks * cin;
+  int kk = 8 * g;
+  int tap = kk / cin, ci = kk - tap * cin;
+  f32x4 acc[NT];
+#pragma unroll
+  for (int t = 0; t < NT; ++t) acc[t] = f32x4{0.f, 0.f, 0.f, 0.f};
+  const uint16_t *wrow[NT];
+#pragma unroll
+  for (int t = 0; t < NT; ++t) wrow[t] = w + (size_t)(co0 + 16 * t + r16) * kpad;
+  for (int kc = 0; kc < kpad; kc += 32) {
+    u32x4 a = u32x4{0u, 0u, 0u, 0u};
+    if (mvalid && kk < K) {
+      int dz, dy, dx;
+      tap_offset(tap, ks, dz, dy, dx);
+      const int64_t src = shifted(vol, vb, vz, vy, vx, dz, dy, dx);
+      if (src >= 0) a = ld16(x + src * cin + ci);
+    }
+    const bf16x8 av = __builtin_bit_cast(bf16x8, a);
+#pragma unroll
+    for (int t = 0; t < NT; ++t) {
+      const bf16x8 bv = __builtin_bit_cast(bf16x8, ld16(wrow[t] + kc + 8 * g));
+      acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, bv, acc[t], 0, 0, 0);
+    }
+    kk += 32;
+    ci += 32;
+    while (ci >= cin) {
+      ci -= cin;
+      ++tap;
+    }
+  }
+#pragma unroll
+  for (int t = 0; t < NT; ++t) {
+    const int co = co0 + 16 * t + r16;
+    const float bb = bias ? bias[co] : 0.f;
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      const uint32_t mo = m0 + (uint32_t)(4 * g + e);
+      if (mo >= vol.vox) continue;
+      float val = acc[t][e] + bb;
+      if (relu) val = val > 0.f ? val : 0.f;
+      y[(size_t)mo * cout + co] = to_bf16(val);
+    }
+  }
+}
+
+// wt[ci][tap'*cout + co] = w[co][tap*cin + ci], tap' = taps-1-tap; zero padding columns
+__global__ void k_wflip(const uint16_t *__restrict__ w, uint16_t *__restrict__ wt, int cin, int cout, int taps,
+                        int kpad_in, int kpad_out) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t total = (int64_t)cin * kpad_out;
+  if (i >= total) return;
+  const int ci = (int)(i / kpad_out), k = (int)(i % kpad_out);
+  uint16_t val = 0;
+  if (k < taps * cout) {
+    const int tp = k / cout, co = k % cout;
+    const int tap = taps - 1 - tp;
+    val = w[(size_t)co * kpad_in + tap * cin + ci];
+  }
+  wt[i] = val;
+}
+
+// ---------------------------------------------------------------------------
+// Weight gradient.  Workgroup (p, tap, cot): voxels [p*vblock, (p+1)*vblock),
+// one tap, output channels 16*cot .. +15; 2 waves, each taking 32 of every 64
+// voxels.  part[((p*2 + wave)*taps + tap)*cout + co][ci].
+// ---------------------------------------------------------------------------
+constexpr int kWgWaves = 2;
+constexpr int kLd = 40;        // LDS row stride (bf16): 80 bytes, conflict-free 16-byte reads
+constexpr int kMaxCinTiles = 16;  // cin <= 256
+
+template <int MAXT>
+__global__ void __launch_bounds__(64 * kWgWaves)
+    k_conv_wgrad(const uint16_t *__restrict__ x, const uint16_t *__restrict__ dy, float *__restrict__ part,
+                 const Vol vol, int cin, int cout, int ks, uint32_t vblock) {
+  extern __shared__ uint16_t lds[];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int g = lane >> 4, r16 = lane & 15;
+  const int p = blockIdx.x, tap = blockIdx.y, co0 = blockIdx.z * 16;
+  const int taps = ks * ks * ks;
+  const int nt = cin / 16;
+  uint16_t *dyT = lds + (size_t)wave * (16 + cin) * kLd;  // [16][kLd]
+  uint16_t *xT = dyT + 16 * kLd;                           // [cin][kLd]
+  int dz, dyo, dx;
+  tap_offset(tap, ks, dz, dyo, dx);
+  const uint32_t v_begin = (uint32_t)p * vblock;
+  uint32_t v_end = v_begin + vblock;
+  if (v_end > vol.vox) v_end = vol.vox;
+  f32x4 acc[MAXT];
+#pragma unroll
+  for (int t = 0; t < MAXT; ++t) acc[t] = f32x4{0.f, 0.f, 0.f, 0.f};
+  const int groups = cin / 8;            // 8-channel groups per voxel row
+  const int pieces = 32 * groups;        // 16-byte pieces of x per 32 voxels
+  for (uint32_t base = v_begin; base < v_end; base += 32u * kWgWaves) {
+    const uint32_t c0 = base + 32u * (uint32_t)wave;  // this wave's 32 voxels
+    // dy^T: lane -> voxel c0 + lane/2, channels co0 + 8*(lane&1) .. +7
+    {
+      const uint32_t v = c0 + (uint32_t)(lane >> 1);
+      u32x4 val = u32x4{0u, 0u, 0u, 0u};
+      if (v < v_end) val = ld16(dy + (size_t)v * cout + co0 + 8 * (lane & 1));
+      const uint16_t *h = reinterpret_cast<const uint16_t *>(&val);
+#pragma unroll
+      for (int e = 0; e < 8; ++e) dyT[(8 * (lane & 1) + e) * kLd + (lane >> 1)] = h[e];
+    }
+    // x^T of the tap-shifted voxels: piece -> voxel piece % 32, channel group piece / 32
+    for (int pc = lane; pc < pieces; pc += 64) {
+      const int vv = pc & 31, grp = pc >> 5;
+      const uint32_t v = c0 + (uint32_t)vv;
+      u32x4 val = u32x4{0u, 0u, 0u, 0u};
+      if (v < v_end) {
+        int b, z, yy, xx;
+        decode(vol, v, b, z, yy, xx);
+        const int64_t src = shifted(vol, b, z, yy, xx, dz, dyo, dx);
+        if (src >= 0) val = ld16(x + src * cin + 8 * grp);
+      }
+      const uint16_t *h = reinterpret_cast<const uint16_t *>(&val);
+#pragma unroll
+      for (int e = 0; e < 8; ++e) xT[(8 * grp + e) * kLd + vv] = h[e];
+    }
+    __syncthreads();
+    const bf16x8 av = __builtin_bit_cast(bf16x8, *reinterpret_cast<const u32x4 *>(dyT + r16 * kLd + 8 * g));
+#pragma unroll
+    for (int t = 0; t < MAXT; ++t) {
+      if (t < nt) {
+        const bf16x8 bv =
+            __builtin_bit_cast(bf16x8, *reinterpret_cast<const u32x4 *>(xT + (16 * t + r16) * kLd + 8 * g));
+        acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, bv, acc[t], 0, 0, 0);
+      }
+    }
+    __syncthreads();
+  }
+  float *out = part + ((((size_t)p * kWgWaves + wave) * taps + tap) * cout + co0) * cin;
+#pragma unroll
+  for (int t = 0; t < MAXT; ++t) {
+    if (t < nt) {
+#pragma unroll
+      for (int e = 0; e < 4; ++e) out[(size_t)(4 * g + e) * cin + 16 * t + r16] = acc[t][e];
+    }
+  }
+}
+
+// dw[co][tap*cin + ci] = sum over the partial sets, in order; padding columns 0
+__global__ void k_wgrad_fold(const float *__restrict__ part, float *__restrict__ dw, int nsets, int taps, int cin,
+                             int cout, int kpad) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (int64_t)cout * kpad) return;
+  const int co = (int)(i / kpad), k = (int)(i % kpad);
+  float s = 0.f;
+  if (k < taps * cin) {
+    const int tap = k / cin, ci = k % cin;
+    const size_t stride = (size_t)taps * cout * cin;
+    const size_t off = ((size_t)tap * cout + co) * cin + ci;
+    for (int q = 0; q < nsets; ++q) s += part[q * stride + off];
+  }
+  dw[i] = s;
+}
+
+int check_conv(int32_t n, int32_t d, int32_t h, int32_t wd, int32_t cin, int32_t cout, int32_t ks) {
+  if (n < 1 || d < 1 || h < 1 || wd < 1) return cfail(LBM3D_EINVAL, "volume extents must be >= 1");
+  if (ks != 1 && ks != 3) return cfail(LBM3D_EINVAL, "kernel size must be 1 or 3");
+  if (cin < 16 || cin > 256 || cin % 16 || cout < 16 || cout > 256 || cout % 16)
+    return cfail(LBM3D_EINVAL, "channel counts must be multiples of 16 in [16, 256]");
+  if ((int64_t)n * d * h * wd * (cin > cout ? cin : cout) >= ((int64_t)1 << 31))
+    return cfail(LBM3D_ESIZE, "tensor exceeds 2^31 elements");
+  return LBM3D_OK;
+}
+
+constexpr int kWgTargetBlocks = 2048;
+
+void wgrad_plan(int32_t n, int32_t d, int32_t h, int32_t wd, int32_t cout, int32_t ks, int &parts, uint32_t &vblock) {
+  const int64_t vox = (int64_t)n * d * h * wd;
+  const int taps = ks * ks * ks;
+  const int64_t per_part = (int64_t)taps * (cout / 16);
+  int64_t P = (kWgTargetBlocks + per_part - 1) / per_part;
+  const int64_t step = 32 * kWgWaves;
+  const int64_t max_parts = (vox + step - 1) / step;
+  if (P > max_parts) P = max_parts;
+  if (P < 1) P = 1;
+  int64_t vb = (vox + P - 1) / P;
+  vb = (vb + step - 1) / step * step;
+  parts = (int)((vox + vb - 1) / vb);
+  vblock = (uint32_t)vb;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *lbm3d_conv_last_error(void) { return c_err; }
+
+int32_t lbm3d_conv_kpad(int32_t cin, int32_t ks) {
+  if (cin < 1 || (ks != 1 && ks != 3)) return 0;
+  const int32_t k = ks * ks * ks * cin;
+  return (k + 31) / 32 * 32;
+}
+
+int lbm3d_conv3d_fwd(void *stream, const uint16_t *x, const uint16_t *w, const float *bias, uint16_t *y, int32_t n,
+                     int32_t d, int32_t h, int32_t wd, int32_t cin, int32_t cout, int32_t ks, int32_t relu) {
+  int rc = check_conv(n, d, h, wd, cin, cout, ks);
+  if (rc) return rc;
+  if (!x || !w || !y) return cfail(LBM3D_EINVAL, "NULL tensor");
+  const Vol vol = make_vol(n, d, h, wd);
+  const int kpad = lbm3d_conv_kpad(cin, ks);
+  hipStream_t s = (hipStream_t)stream;
+  const unsigned gx = (vol.vox + 63u) / 64u;
+  if (cout % 64 == 0)
+    k_conv_fwd<4><<<dim3(gx, cout / 64), 256, 0, s>>>(x, w, bias, y, vol, cin, cout, ks, kpad, relu);
+  else if (cout % 32 == 0)
+    k_conv_fwd<2><<<dim3(gx, cout / 32), 256, 0, s>>>(x, w, bias, y, vol, cin, cout, ks, kpad, relu);
+  else
+    k_conv_fwd<1><<<dim3(gx, cout / 16), 256, 0, s>>>(x, w, bias, y, vol, cin, cout, ks, kpad, relu);
+  return ccheck("lbm3d_conv3d_fwd");
+}
+
+int lbm3d_conv3d_wflip(void *stream, const uint16_t *w, uint16_t *wt, int32_t cin, int32_t cout, int32_t ks) {
+  int rc = check_conv(1, 1, 1, 1, cin, cout, ks);
+  if (rc) return rc;
+  if (!w || !wt) return cfail(LBM3D_EINVAL, "NULL tensor");
+  const int taps = ks * ks * ks;
+  const int kin = lbm3d_conv_kpad(cin, ks), kout = lbm3d_conv_kpad(cout, ks);
+  const int64_t total = (int64_t)cin * kout;
+  k_wflip<<<(unsigned)((total + 255) / 256), 256, 0, (hipStream_t)stream>>>(w, wt, cin, cout, taps, kin, kout);
+  return ccheck("lbm3d_conv3d_wflip");
+}
+
+int lbm3d_conv3d_dgrad(void *stream, const uint16_t *dy, const uint16_t *wt, uint16_t *dx, int32_t n, int32_t d,
+                       int32_t h, int32_t wd, int32_t cin, int32_t cout, int32_t ks) {
+  // a forward convolution from cout to cin channels with the flipped weights
+  return lbm3d_conv3d_fwd(stream, dy, wt, nullptr, dx, n, d, h, wd, cout, cin, ks, 0);
+}
+
+size_t lbm3d_conv3d_wgrad_workspace_bytes(int32_t n, int32_t d, int32_t h, int32_t wd, int32_t cin, int32_t cout,
+                                          int32_t ks) {
+  if (check_conv(n, d, h, wd, cin, cout, ks)) return 0;
+  int parts;
+  uint32_t vb;
+  wgrad_plan(n, d, h, wd, cout, ks, parts, vb);
+  return (size_t)parts * kWgWaves * ks * ks * ks * cout * cin * sizeof(float);
+}
+
+int lbm3d_conv3d_wgrad(void *stream, const uint16_t *x, const uint16_t *dy, float *dw, int32_t n, int32_t d, int32_t h,
+                       int32_t wd, int32_t cin, int32_t cout, int32_t ks, void *workspace, size_t workspace_bytes) {
+  int rc = check_conv(n, d, h, wd, cin, cout, ks);
+  if (rc) return rc;
+  if (!x || !dy || !dw || !workspace) return cfail(LBM3D_EINVAL, "NULL tensor or workspace");
+  if (workspace_bytes < lbm3d_conv3d_wgrad_workspace_bytes(n, d, h, wd, cin, cout, ks))
+    return cfail(LBM3D_EINVAL, "wgrad workspace too small");
+  const Vol vol = make_vol(n, d, h, wd);
+  int parts;
+  uint32_t vb;
+  wgrad_plan(n, d, h, wd, cout, ks, parts, vb);
+  const int taps = ks * ks * ks;
+  const size_t lds = (size_t)kWgWaves * (16 + cin) * kLd * sizeof(uint16_t);
+  hipStream_t s = (hipStream_t)stream;
+  const dim3 grid((unsigned)parts, (unsigned)taps, (unsigned)(cout / 16));
+  float *part = (float *)workspace;
+  if (cin <= 64)
+    k_conv_wgrad<4><<<grid, 64 * kWgWaves, lds, s>>>(x, dy, part, vol, cin, cout, ks, vb);
+  else if (cin <= 128)
+    k_conv_wgrad<8><<<grid, 64 * kWgWaves, lds, s>>>(x, dy, part, vol, cin, cout, ks, vb);
+  else
+    k_conv_wgrad<kMaxCinTiles><<<grid, 64 * kWgWaves, lds, s>>>(x, dy, part, vol, cin, cout, ks, vb);
+  if ((rc = ccheck("lbm3d_conv3d_wgrad"))) return rc;
+  const int kpad = lbm3d_conv_kpad(cin, ks);
+  const int64_t total = (int64_t)cout * kpad;
+  k_wgrad_fold<<<(unsigned)((total + 255) / 256), 256, 0, s>>>(part, dw, parts * kWgWaves, taps, cin, cout, kpad);
+  return ccheck("lbm3d_conv3d_wgrad/fold");
+}
+
+}  // extern "C"

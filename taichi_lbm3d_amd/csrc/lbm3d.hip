@@ -20,6 +20,7 @@
 #include <hip/hip_fp16.h>
 #include <hip/hip_bf16.h>
 
+#include <atomic>
 #include <cstdint>
 #include <cstdio>
 #include <cstring>
@@ -110,17 +111,24 @@ struct Phys {
   R u_bc[6][3];
 };
 
-struct KP {
-  lbm3d_params p;          // integer fields are read on the device; its doubles are not
-  Phys<float> pf;
-  Phys<double> pd;
-  uint32_t S;              // slot stride
+// Grid geometry of one sample: everything the kernels derive from (nx, ny, nz).
+struct Geo {
+  int32_t nx, ny, nz;
   uint32_t W;              // 32-cell words per grid row
   uint32_t nyz;            // ny * nz
   uint32_t m_nz, s_nz;     // c / nz  == (c * m_nz) >> s_nz   for c < 2^31
   uint32_t m_nyz, s_nyz;   // c / nyz == (c * m_nyz) >> s_nyz
   uint32_t plane[6];       // cells per face plane
+  int32_t face_u_off[6];   // first face_u element of each pressure face's plane
   uint32_t n_row_words;    // nx * ny * W
+};
+
+struct KP {
+  lbm3d_params p;          // integer fields are read on the device; its doubles are not
+  Phys<float> pf;
+  Phys<double> pd;
+  Geo geo;
+  uint32_t S;              // slot stride
 };
 
 // Exact division of 31-bit numerators by d: m = ceil(2^(31+l) / d), l = ceil(log2 d).
@@ -130,6 +138,23 @@ void magic(uint32_t d, uint32_t &m, uint32_t &s) {
   const uint64_t two = 1ull << (31 + l);
   m = (uint32_t)((two + d - 1) / d);
   s = 31 + l;
+}
+
+Geo make_geo(int32_t nx, int32_t ny, int32_t nz, const int32_t face_u_off[6]) {
+  Geo g;
+  g.nx = nx;
+  g.ny = ny;
+  g.nz = nz;
+  g.W = (uint32_t)(nz + 31) / 32;
+  g.nyz = (uint32_t)ny * (uint32_t)nz;
+  magic((uint32_t)nz, g.m_nz, g.s_nz);
+  magic(g.nyz, g.m_nyz, g.s_nyz);
+  g.plane[0] = g.plane[1] = (uint32_t)ny * nz;
+  g.plane[2] = g.plane[3] = (uint32_t)nx * nz;
+  g.plane[4] = g.plane[5] = (uint32_t)nx * ny;
+  for (int f = 0; f < 6; ++f) g.face_u_off[f] = face_u_off[f];
+  g.n_row_words = (uint32_t)nx * (uint32_t)ny * g.W;
+  return g;
 }
 
 template <typename R>
@@ -153,14 +178,7 @@ KP make_kp(const lbm3d_params &p) {
   fill_phys(k.pf, p);
   fill_phys(k.pd, p);
   k.S = slot_stride((uint32_t)(p.n_cells > 0 ? p.n_cells : 0));
-  k.W = (uint32_t)(p.nz + 31) / 32;
-  k.nyz = (uint32_t)p.ny * (uint32_t)p.nz;
-  magic((uint32_t)p.nz, k.m_nz, k.s_nz);
-  magic(k.nyz, k.m_nyz, k.s_nyz);
-  k.plane[0] = k.plane[1] = (uint32_t)p.ny * p.nz;
-  k.plane[2] = k.plane[3] = (uint32_t)p.nx * p.nz;
-  k.plane[4] = k.plane[5] = (uint32_t)p.nx * p.ny;
-  k.n_row_words = (uint32_t)p.nx * (uint32_t)p.ny * k.W;
+  k.geo = make_geo(p.nx, p.ny, p.nz, p.face_u_off);
   return k;
 }
 
@@ -172,13 +190,13 @@ __device__ __forceinline__ uint32_t fdiv(uint32_t c, uint32_t m, uint32_t s) {
   return (uint32_t)(((uint64_t)c * m) >> s);
 }
 
-__device__ __forceinline__ void coords(const KP &kp, uint32_t c, int &i, int &j, int &k) {
-  const uint32_t a = fdiv(c, kp.m_nyz, kp.s_nyz);
-  const uint32_t r = c - a * kp.nyz;
-  const uint32_t b = fdiv(r, kp.m_nz, kp.s_nz);
+__device__ __forceinline__ void coords(const Geo &gm, uint32_t c, int &i, int &j, int &k) {
+  const uint32_t a = fdiv(c, gm.m_nyz, gm.s_nyz);
+  const uint32_t r = c - a * gm.nyz;
+  const uint32_t b = fdiv(r, gm.m_nz, gm.s_nz);
   i = (int)a;
   j = (int)b;
-  k = (int)(r - b * (uint32_t)kp.p.nz);
+  k = (int)(r - b * (uint32_t)gm.nz);
 }
 
 // periodic_index, SP:247-257: -1 -> n-1 and n -> 0 on every axis, whatever the BC mode
@@ -233,14 +251,14 @@ __device__ __forceinline__ int32_t id_in_word(uint4 w, int k) {
 }
 
 // ids of cells (k-1, k, k+1) of one row, k-1 and k+1 wrapped, from one word
-__device__ __forceinline__ void row3(const Rows &rb, const KP &kp, uint32_t row, int k, int32_t &lo,
+__device__ __forceinline__ void row3(const Rows &rb, const Geo &gm, uint32_t row, int k, int32_t &lo,
                                      int32_t &mid, int32_t &hi) {
-  const uint4 w = rb.word(row, kp.W, k);
+  const uint4 w = rb.word(row, gm.W, k);
   const uint32_t b = (uint32_t)k & 31u;
   const uint32_t below = w.y + __popc(w.x & ((1u << b) - 1u));
   const uint32_t fm = (w.x >> b) & 1u;
   mid = fm ? (int32_t)below : -1;
-  const bool last = b == 31u || k == kp.p.nz - 1;
+  const bool last = b == 31u || k == gm.nz - 1;
   const bool up_fluid = (w.x >> ((b + 1u) & 31u)) & 1u;
   hi = last ? (int32_t)w.w : (up_fluid ? (int32_t)(below + fm) : -1);
   const bool down_fluid = (w.x >> ((b - 1u) & 31u)) & 1u;
@@ -250,41 +268,41 @@ __device__ __forceinline__ void row3(const Rows &rb, const KP &kp, uint32_t row,
 // Compact ids of the 18 neighbours (index q-1), -1 where the neighbour is
 // not fluid: the push target of streaming1 (SP:264-266) or its bounce-back
 // (SP:267-268).
-__device__ __forceinline__ void neighbor_ids(const Rows &rb, const KP &kp, int i, int j, int k, int32_t nb[18]) {
-  const uint32_t ny = (uint32_t)kp.p.ny;
-  const uint32_t ip = (uint32_t)wrap(i + 1, kp.p.nx), im = (uint32_t)wrap(i - 1, kp.p.nx);
-  const uint32_t jp = (uint32_t)wrap(j + 1, kp.p.ny), jm = (uint32_t)wrap(j - 1, kp.p.ny);
+__device__ __forceinline__ void neighbor_ids(const Rows &rb, const Geo &gm, int i, int j, int k, int32_t nb[18]) {
+  const uint32_t ny = (uint32_t)gm.ny;
+  const uint32_t ip = (uint32_t)wrap(i + 1, gm.nx), im = (uint32_t)wrap(i - 1, gm.nx);
+  const uint32_t jp = (uint32_t)wrap(j + 1, gm.ny), jm = (uint32_t)wrap(j - 1, gm.ny);
   const uint32_t ii = (uint32_t)i, jj = (uint32_t)j;
   int32_t a, b, c;
   // own row: q5 (0,0,1), q6 (0,0,-1)
-  row3(rb, kp, ii * ny + jj, k, a, b, c);
+  row3(rb, gm, ii * ny + jj, k, a, b, c);
   nb[4] = c;
   nb[5] = a;
   // row (i+1, j): q1 (1,0,0), q11 (1,0,1), q13 (1,0,-1)
-  row3(rb, kp, ip * ny + jj, k, a, b, c);
+  row3(rb, gm, ip * ny + jj, k, a, b, c);
   nb[0] = b;
   nb[10] = c;
   nb[12] = a;
   // row (i-1, j): q2 (-1,0,0), q12 (-1,0,-1), q14 (-1,0,1)
-  row3(rb, kp, im * ny + jj, k, a, b, c);
+  row3(rb, gm, im * ny + jj, k, a, b, c);
   nb[1] = b;
   nb[11] = a;
   nb[13] = c;
   // row (i, j+1): q3 (0,1,0), q15 (0,1,1), q17 (0,1,-1)
-  row3(rb, kp, ii * ny + jp, k, a, b, c);
+  row3(rb, gm, ii * ny + jp, k, a, b, c);
   nb[2] = b;
   nb[14] = c;
   nb[16] = a;
   // row (i, j-1): q4 (0,-1,0), q16 (0,-1,-1), q18 (0,-1,1)
-  row3(rb, kp, ii * ny + jm, k, a, b, c);
+  row3(rb, gm, ii * ny + jm, k, a, b, c);
   nb[3] = b;
   nb[15] = a;
   nb[17] = c;
   // diagonal rows: q7 (1,1,0), q8 (-1,-1,0), q9 (1,-1,0), q10 (-1,1,0)
-  nb[6] = id_in_word(rb.word(ip * ny + jp, kp.W, k), k);
-  nb[7] = id_in_word(rb.word(im * ny + jm, kp.W, k), k);
-  nb[8] = id_in_word(rb.word(ip * ny + jm, kp.W, k), k);
-  nb[9] = id_in_word(rb.word(im * ny + jp, kp.W, k), k);
+  nb[6] = id_in_word(rb.word(ip * ny + jp, gm.W, k), k);
+  nb[7] = id_in_word(rb.word(im * ny + jm, gm.W, k), k);
+  nb[8] = id_in_word(rb.word(ip * ny + jm, gm.W, k), k);
+  nb[9] = id_in_word(rb.word(im * ny + jp, gm.W, k), k);
 }
 
 // ---------------------------------------------------------------------------
@@ -498,22 +516,22 @@ __device__ __forceinline__ R feq_shift(int q, R rho, R ux, R uy, R uz, R uu) {
 // which contains the node (Boundary_condition applies faces in that order,
 // SP:274-370, so later faces overwrite earlier ones).
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ int face_owner(const lbm3d_params &p, int i, int j, int k) {
+__device__ __forceinline__ int face_owner(const lbm3d_params &p, const Geo &gm, int i, int j, int k) {
   int owner = -1;
 #pragma unroll
   for (int f = 0; f < 6; ++f) {
     const int m = p.bc_mode[f];
     if (m != 1 && m != 2) continue;
     const int pos = f < 2 ? i : (f < 4 ? j : k);
-    const int n = f < 2 ? p.nx : (f < 4 ? p.ny : p.nz);
+    const int n = f < 2 ? gm.nx : (f < 4 ? gm.ny : gm.nz);
     if (pos == ((f & 1) ? n - 1 : 0)) owner = f;
   }
   return owner;
 }
 
 // index of (i,j,k) in face f's plane
-__device__ __forceinline__ uint32_t plane_index(const lbm3d_params &p, int f, int i, int j, int k) {
-  return f < 2 ? (uint32_t)j * p.nz + k : (f < 4 ? (uint32_t)i * p.nz + k : (uint32_t)i * p.ny + j);
+__device__ __forceinline__ uint32_t plane_index(const Geo &gm, int f, int i, int j, int k) {
+  return f < 2 ? (uint32_t)j * gm.nz + k : (f < 4 ? (uint32_t)i * gm.nz + k : (uint32_t)i * gm.ny + j);
 }
 
 // Boundary_condition (SP:272-370) for a face-owned node: every population is
@@ -521,7 +539,7 @@ __device__ __forceinline__ uint32_t plane_index(const lbm3d_params &p, int f, in
 // previous-step velocity, or 0 when the inward neighbour is solid (its v is 0,
 // SP:390-392).  Velocity (mode 2): rho 1 and the face velocity.
 template <typename R>
-__device__ __forceinline__ void face_populations(R v[kQ], const KP &kp, int owner, bool inward_solid,
+__device__ __forceinline__ void face_populations(R v[kQ], const KP &kp, const Geo &gm, int owner, bool inward_solid,
                                                  const R *__restrict__ face_u, uint32_t pidx) {
   const lbm3d_params &p = kp.p;
   const Phys<R> &ph = phys<R>(kp);
@@ -531,8 +549,8 @@ __device__ __forceinline__ void face_populations(R v[kQ], const KP &kp, int owne
     if (inward_solid) {
       ux = uy = uz = R(0);
     } else {
-      const R *fu = face_u + p.face_u_off[owner];
-      const uint32_t P = kp.plane[owner];
+      const R *fu = face_u + gm.face_u_off[owner];
+      const uint32_t P = gm.plane[owner];
       ux = fu[pidx];
       uy = fu[P + pidx];
       uz = fu[2 * P + pidx];
@@ -567,7 +585,7 @@ struct CellCtx {
 };
 
 template <typename T, bool ODD>
-__device__ __forceinline__ void read_cell(const Slots<T> &sl, const KP &kp, const int32_t *__restrict__ cells,
+__device__ __forceinline__ void read_cell(const Slots<T> &sl, const Geo &gm, const int32_t *__restrict__ cells,
                                           const Rows &rb, uint32_t x, typename St<T>::R v[kQ], uint32_t off[kQ],
                                           CellCtx &cc) {
   const uint32_t xb = x * (uint32_t)sizeof(T);
@@ -576,10 +594,10 @@ __device__ __forceinline__ void read_cell(const Slots<T> &sl, const KP &kp, cons
 #pragma unroll
     for (int q = 0; q < kQ; ++q) v[q] = sl.ld_slot(q, xb);
   } else {
-    coords(kp, (uint32_t)cells[x], cc.i, cc.j, cc.k);
+    coords(gm, (uint32_t)cells[x], cc.i, cc.j, cc.k);
     cc.have_coords = true;
     int32_t nb[18];
-    neighbor_ids(rb, kp, cc.i, cc.j, cc.k, nb);
+    neighbor_ids(rb, gm, cc.i, cc.j, cc.k, nb);
     off[0] = xb;
 #pragma unroll
     for (int q = 1; q < kQ; ++q) {
@@ -599,39 +617,36 @@ __device__ __forceinline__ uint32_t face_record(const uint32_t *__restrict__ fac
   return (face_bits[x >> 4] >> ((x & 15u) * 2u)) & 3u;
 }
 
+// One fluid cell's time step: read (with the previous step's face overwrite),
+// macros, pressure-face state, collision, write back in place.
 template <typename T, bool ODD, bool FORCED>
-__global__ void __launch_bounds__(kBlock, LBM3D_STEP_MINB)
-    k_step(const KP kp, T *__restrict__ g, const int32_t *__restrict__ cells, const uint4 *__restrict__ rows,
-           const uint32_t *__restrict__ face_bits, typename St<T>::R *__restrict__ face_u, int first) {
+__device__ __forceinline__ void step_cell(const KP &kp, const Geo &gm, const Slots<T> &sl, const Rows &rb,
+                                          const int32_t *__restrict__ cells, const uint32_t *__restrict__ face_bits,
+                                          typename St<T>::R *__restrict__ face_u, uint32_t x, int first) {
   using R = typename St<T>::R;
-  const uint32_t x = logical_block<ODD ? LBM3D_XCD_REMAP_ODD : LBM3D_XCD_REMAP_EVEN,
-                                   !ODD && LBM3D_REVERSE_EVEN>() * kBlock + threadIdx.x;
   const lbm3d_params &p = kp.p;
-  if (x >= (uint32_t)p.n_cells) return;
   const Phys<R> &ph = phys<R>(kp);
-  const Slots<T> sl(g, kp.S);
-  const Rows rb(rows, kp.n_row_words);
   R v[kQ];
   uint32_t off[kQ];
   CellCtx cc;
-  read_cell<T, ODD>(sl, kp, cells, rb, x, v, off, cc);
+  read_cell<T, ODD>(sl, gm, cells, rb, x, v, off, cc);
 
   const uint32_t fr = face_record(face_bits, x);
   int owner = -1;
   uint32_t pidx = 0;
   if (fr & 1u) {
-    if (!cc.have_coords) coords(kp, (uint32_t)cells[x], cc.i, cc.j, cc.k);
-    owner = face_owner(p, cc.i, cc.j, cc.k);
-    pidx = plane_index(p, owner, cc.i, cc.j, cc.k);
-    if (!first) face_populations<R>(v, kp, owner, (fr & 2u) != 0, face_u, pidx);
+    if (!cc.have_coords) coords(gm, (uint32_t)cells[x], cc.i, cc.j, cc.k);
+    owner = face_owner(p, gm, cc.i, cc.j, cc.k);
+    pidx = plane_index(gm, owner, cc.i, cc.j, cc.k);
+    if (!first) face_populations<R>(v, kp, gm, owner, (fr & 2u) != 0, face_u, pidx);
   }
   R P[10], D[10];
   pair_sums<R>(v, P, D);
   const Macro<R> m = macros_pd<R>(v[0], P, D, ph, first != 0);
   if (owner >= 0 && p.bc_mode[owner] == 1 && !(fr & 2u)) {
     // the pressure face of the next step uses this step's v (SP:281)
-    R *fu = face_u + p.face_u_off[owner];
-    const uint32_t PL = kp.plane[owner];
+    R *fu = face_u + gm.face_u_off[owner];
+    const uint32_t PL = gm.plane[owner];
     fu[pidx] = m.ux;
     fu[PL + pidx] = m.uy;
     fu[2 * PL + pidx] = m.uz;
@@ -649,28 +664,23 @@ __global__ void __launch_bounds__(kBlock, LBM3D_STEP_MINB)
   }
 }
 
+// rho and v of one fluid cell as streaming3 leaves them after the last step
 template <typename T, bool ODD>
-__global__ void __launch_bounds__(kBlock)
-    k_macros(const KP kp, const T *__restrict__ g, const int32_t *__restrict__ cells,
-             const uint4 *__restrict__ rows, const uint32_t *__restrict__ face_bits,
-             const typename St<T>::R *__restrict__ face_u, int first, typename St<T>::R *__restrict__ rho_out,
-             typename St<T>::R *__restrict__ v_out) {
+__device__ __forceinline__ void macros_cell(const KP &kp, const Geo &gm, const Slots<T> &sl, const Rows &rb,
+                                            const int32_t *__restrict__ cells, const uint32_t *__restrict__ face_bits,
+                                            const typename St<T>::R *__restrict__ face_u, uint32_t x, int first,
+                                            typename St<T>::R *__restrict__ rho_out,
+                                            typename St<T>::R *__restrict__ v_out, uint32_t N) {
   using R = typename St<T>::R;
-  const uint32_t x = blockIdx.x * kBlock + threadIdx.x;
-  const lbm3d_params &p = kp.p;
-  const uint32_t N = (uint32_t)p.n_cells;
-  if (x >= N) return;
-  const Slots<T> sl(g, kp.S);
-  const Rows rb(rows, kp.n_row_words);
   R v[kQ];
   uint32_t off[kQ];
   CellCtx cc;
-  read_cell<T, ODD>(sl, kp, cells, rb, x, v, off, cc);
+  read_cell<T, ODD>(sl, gm, cells, rb, x, v, off, cc);
   const uint32_t fr = face_record(face_bits, x);
   if ((fr & 1u) && !first) {
-    if (!cc.have_coords) coords(kp, (uint32_t)cells[x], cc.i, cc.j, cc.k);
-    const int owner = face_owner(p, cc.i, cc.j, cc.k);
-    face_populations<R>(v, kp, owner, (fr & 2u) != 0, face_u, plane_index(p, owner, cc.i, cc.j, cc.k));
+    if (!cc.have_coords) coords(gm, (uint32_t)cells[x], cc.i, cc.j, cc.k);
+    const int owner = face_owner(kp.p, gm, cc.i, cc.j, cc.k);
+    face_populations<R>(v, kp, gm, owner, (fr & 2u) != 0, face_u, plane_index(gm, owner, cc.i, cc.j, cc.k));
   }
   R P[10], D[10];
   pair_sums<R>(v, P, D);
@@ -681,16 +691,14 @@ __global__ void __launch_bounds__(kBlock)
   v_out[2 * (size_t)N + x] = m.uz;
 }
 
+// init (SP:160-170) of one fluid cell: equilibrium at (rho0, u0), default rest
 template <typename T>
-__global__ void __launch_bounds__(kBlock)
-    k_init(const KP kp, T *__restrict__ g, const int32_t *__restrict__ cells, typename St<T>::R *__restrict__ face_u,
-           const float *__restrict__ rho0, const float *__restrict__ u0) {
+__device__ __forceinline__ void init_cell(const KP &kp, const Geo &gm, const Slots<T> &sl,
+                                          const int32_t *__restrict__ cells, typename St<T>::R *__restrict__ face_u,
+                                          const float *__restrict__ rho0, const float *__restrict__ u0, uint32_t x,
+                                          uint32_t N) {
   using R = typename St<T>::R;
-  const uint32_t x = blockIdx.x * kBlock + threadIdx.x;
   const lbm3d_params &p = kp.p;
-  const uint32_t N = (uint32_t)p.n_cells;
-  if (x >= N) return;
-  const Slots<T> sl(g, kp.S);
   R rho = R(1), ux = R(0), uy = R(0), uz = R(0);
   if (rho0) rho = (R)rho0[x];
   if (u0) {
@@ -703,15 +711,124 @@ __global__ void __launch_bounds__(kBlock)
 #pragma unroll
   for (int q = 0; q < kQ; ++q) sl.st_slot(q, xb, feq_shift<R>(q, rho, ux, uy, uz, uu));
   int i, j, k;
-  coords(kp, (uint32_t)cells[x], i, j, k);
-  const int owner = face_owner(p, i, j, k);
+  coords(gm, (uint32_t)cells[x], i, j, k);
+  const int owner = face_owner(p, gm, i, j, k);
   if (owner >= 0 && p.bc_mode[owner] == 1) {
-    R *fu = face_u + p.face_u_off[owner];
-    const uint32_t PL = kp.plane[owner], pi = plane_index(p, owner, i, j, k);
+    R *fu = face_u + gm.face_u_off[owner];
+    const uint32_t PL = gm.plane[owner], pi = plane_index(gm, owner, i, j, k);
     fu[pi] = ux;
     fu[PL + pi] = uy;
     fu[2 * PL + pi] = uz;
   }
+}
+
+template <typename T, bool ODD, bool FORCED>
+__global__ void __launch_bounds__(kBlock, LBM3D_STEP_MINB)
+    k_step(const KP kp, T *__restrict__ g, const int32_t *__restrict__ cells, const uint4 *__restrict__ rows,
+           const uint32_t *__restrict__ face_bits, typename St<T>::R *__restrict__ face_u, int first) {
+  const uint32_t x = logical_block<ODD ? LBM3D_XCD_REMAP_ODD : LBM3D_XCD_REMAP_EVEN,
+                                   !ODD && LBM3D_REVERSE_EVEN>() * kBlock + threadIdx.x;
+  if (x >= (uint32_t)kp.p.n_cells) return;
+  const Slots<T> sl(g, kp.S);
+  const Rows rb(rows, kp.geo.n_row_words);
+  step_cell<T, ODD, FORCED>(kp, kp.geo, sl, rb, cells, face_bits, face_u, x, first);
+}
+
+template <typename T, bool ODD>
+__global__ void __launch_bounds__(kBlock)
+    k_macros(const KP kp, const T *__restrict__ g, const int32_t *__restrict__ cells,
+             const uint4 *__restrict__ rows, const uint32_t *__restrict__ face_bits,
+             const typename St<T>::R *__restrict__ face_u, int first, typename St<T>::R *__restrict__ rho_out,
+             typename St<T>::R *__restrict__ v_out) {
+  const uint32_t x = blockIdx.x * kBlock + threadIdx.x;
+  const uint32_t N = (uint32_t)kp.p.n_cells;
+  if (x >= N) return;
+  const Slots<T> sl(g, kp.S);
+  const Rows rb(rows, kp.geo.n_row_words);
+  macros_cell<T, ODD>(kp, kp.geo, sl, rb, cells, face_bits, face_u, x, first, rho_out, v_out, N);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kBlock)
+    k_init(const KP kp, T *__restrict__ g, const int32_t *__restrict__ cells, typename St<T>::R *__restrict__ face_u,
+           const float *__restrict__ rho0, const float *__restrict__ u0) {
+  const uint32_t x = blockIdx.x * kBlock + threadIdx.x;
+  const uint32_t N = (uint32_t)kp.p.n_cells;
+  if (x >= N) return;
+  const Slots<T> sl(g, kp.S);
+  init_cell<T>(kp, kp.geo, sl, cells, face_u, rho0, u0, x, N);
+}
+
+// ---------------------------------------------------------------------------
+// Batches.  Samples of one batch share the physical parameters and face modes;
+// each has its own grid.  Their fluid lists are laid end to end in one slot
+// range, each starting on a kChunk-cell boundary, and every row word holds
+// batch-wide compact ids, so the neighbour lookup and the population accesses
+// are those of a single grid.  chunk_sample[c] names the sample owning cells
+// c*kChunk .. c*kChunk + kChunk-1; a workgroup of B | kChunk cells therefore
+// lies in one sample and reads that sample's descriptor with scalar loads.
+// ---------------------------------------------------------------------------
+constexpr uint32_t kChunk = LBM3D_BATCH_ALIGN;
+
+struct SampleDesc {
+  Geo geo;
+  uint32_t cell_begin, cell_end;    // the sample's fluid cells: compact ids [cell_begin, cell_end)
+  uint32_t chunk_begin, chunk_end;  // its kChunk-cell chunks
+  uint64_t rows_begin;              // its first row word
+};
+
+struct Batch {
+  const SampleDesc *desc;
+  const int32_t *chunk_sample;
+  const uint8_t *active;            // per sample: 0 = skip (retired), or NULL = all active
+};
+
+// Persistent launch: gridDim.x workgroups walk the n_tiles tiles of B cells,
+// forwards or backwards.
+template <typename T, bool ODD, bool FORCED, int B>
+__global__ void __launch_bounds__(B)
+    k_step_batch(const KP kp, const Batch bt, T *__restrict__ g, const int32_t *__restrict__ cells,
+                 const uint4 *__restrict__ rows, const uint32_t *__restrict__ face_bits,
+                 typename St<T>::R *__restrict__ face_u, int first, uint32_t n_tiles, int reverse) {
+  const Slots<T> sl(g, kp.S);
+  for (uint32_t it = blockIdx.x; it < n_tiles; it += gridDim.x) {
+    const uint32_t x0 = (reverse ? n_tiles - 1u - it : it) * (uint32_t)B;
+    const int s = bt.chunk_sample[x0 / kChunk];
+    if (bt.active && !bt.active[s]) continue;
+    const SampleDesc &d = bt.desc[s];
+    const uint32_t x = x0 + threadIdx.x;
+    if (x >= d.cell_end) continue;
+    const Rows rb(rows + d.rows_begin, d.geo.n_row_words);
+    step_cell<T, ODD, FORCED>(kp, d.geo, sl, rb, cells, face_bits, face_u, x, first);
+  }
+}
+
+template <typename T, bool ODD>
+__global__ void __launch_bounds__(kBlock)
+    k_macros_batch(const KP kp, const Batch bt, const T *__restrict__ g, const int32_t *__restrict__ cells,
+                   const uint4 *__restrict__ rows, const uint32_t *__restrict__ face_bits,
+                   const typename St<T>::R *__restrict__ face_u, int first, typename St<T>::R *__restrict__ rho_out,
+                   typename St<T>::R *__restrict__ v_out) {
+  const uint32_t x0 = blockIdx.x * kBlock;
+  const SampleDesc &d = bt.desc[bt.chunk_sample[x0 / kChunk]];
+  const uint32_t x = x0 + threadIdx.x;
+  if (x >= d.cell_end) return;
+  const Slots<T> sl(g, kp.S);
+  const Rows rb(rows + d.rows_begin, d.geo.n_row_words);
+  macros_cell<T, ODD>(kp, d.geo, sl, rb, cells, face_bits, face_u, x, first, rho_out, v_out, (uint32_t)kp.p.n_cells);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kBlock)
+    k_init_batch(const KP kp, const Batch bt, T *__restrict__ g, const int32_t *__restrict__ cells,
+                 typename St<T>::R *__restrict__ face_u, const float *__restrict__ rho0,
+                 const float *__restrict__ u0) {
+  const uint32_t x0 = blockIdx.x * kBlock;
+  const SampleDesc &d = bt.desc[bt.chunk_sample[x0 / kChunk]];
+  const uint32_t x = x0 + threadIdx.x;
+  if (x >= d.cell_end) return;
+  const Slots<T> sl(g, kp.S);
+  init_cell<T>(kp, d.geo, sl, cells, face_u, rho0, u0, x, (uint32_t)kp.p.n_cells);
 }
 
 // ---------------------------------------------------------------------------
@@ -799,14 +916,16 @@ __global__ void __launch_bounds__(kBlock) k_compact_fill(const int8_t *__restric
 }
 
 // rows[(i*ny + j)*W + w] = {fluid mask of cells 32w .. 32w+31, compact id of the
-// first of them, id of the cell before 32w, id of the cell after 32w+31}
-__global__ void __launch_bounds__(kBlock) k_build_rows(const KP kp, const int8_t *__restrict__ solid,
-                                                       const int32_t *__restrict__ d2c, uint4 *__restrict__ rows) {
+// first of them, id of the cell before 32w, id of the cell after 32w+31}, every
+// id shifted by id_offset (the sample's first compact id inside a batch)
+__global__ void __launch_bounds__(kBlock) k_build_rows(const Geo gm, const int8_t *__restrict__ solid,
+                                                       const int32_t *__restrict__ d2c, int32_t id_offset,
+                                                       uint4 *__restrict__ rows) {
   const uint64_t t = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
-  if (t >= kp.n_row_words) return;
-  const uint64_t r = t / kp.W;
-  const uint32_t w = (uint32_t)(t % kp.W);
-  const int nz = kp.p.nz;
+  if (t >= gm.n_row_words) return;
+  const uint64_t r = t / gm.W;
+  const uint32_t w = (uint32_t)(t % gm.W);
+  const int nz = gm.nz;
   const int64_t row0 = (int64_t)r * nz;
   const int64_t c0 = row0 + 32 * w;
   uint32_t mask = 0;
@@ -817,14 +936,16 @@ __global__ void __launch_bounds__(kBlock) k_build_rows(const KP kp, const int8_t
     if (solid[c0 + b] == 0) {
       mask |= 1u << b;
       if (!found) {
-        base = d2c[c0 + b];
+        base = d2c[c0 + b] + id_offset;
         found = true;
       }
     }
   }
   const int left = w == 0 ? nz - 1 : (int)(32 * w) - 1;            // wrap at k = 0
-  const int right = (w == kp.W - 1) ? 0 : (int)(32 * w) + 32;      // wrap after k = nz-1
-  rows[t] = make_uint4(mask, (uint32_t)base, (uint32_t)d2c[row0 + left], (uint32_t)d2c[row0 + right]);
+  const int right = (w == gm.W - 1) ? 0 : (int)(32 * w) + 32;      // wrap after k = nz-1
+  const int32_t il = d2c[row0 + left], ir = d2c[row0 + right];
+  rows[t] = make_uint4(mask, (uint32_t)base, (uint32_t)(il >= 0 ? il + id_offset : -1),
+                       (uint32_t)(ir >= 0 ? ir + id_offset : -1));
 }
 
 // face_bits: 16 cells per word, 2 bits per cell
@@ -833,6 +954,7 @@ __global__ void __launch_bounds__(kBlock) k_build_face_bits(const KP kp, const i
                                                             uint32_t *__restrict__ face_bits) {
   const uint32_t wi = blockIdx.x * kBlock + threadIdx.x;
   const lbm3d_params &p = kp.p;
+  const Geo &gm = kp.geo;
   const uint32_t N = (uint32_t)p.n_cells;
   if (wi >= (N + 15) / 16) return;
   uint32_t word = 0;
@@ -840,14 +962,14 @@ __global__ void __launch_bounds__(kBlock) k_build_face_bits(const KP kp, const i
     const uint32_t x = wi * 16 + b;
     if (x >= N) break;
     int i, j, k;
-    coords(kp, (uint32_t)cells[x], i, j, k);
-    const int owner = face_owner(p, i, j, k);
+    coords(gm, (uint32_t)cells[x], i, j, k);
+    const int owner = face_owner(p, gm, i, j, k);
     if (owner < 0) continue;
     int ii = i, jj = j, kk = k;
-    const int n = owner < 2 ? p.nx : (owner < 4 ? p.ny : p.nz);
+    const int n = owner < 2 ? gm.nx : (owner < 4 ? gm.ny : gm.nz);
     const int inw = (owner & 1) ? n - 2 : 1;
     if (owner < 2) ii = inw; else if (owner < 4) jj = inw; else kk = inw;
-    const bool inward_solid = solid[((int64_t)ii * p.ny + jj) * p.nz + kk] > 0;  // SP:278
+    const bool inward_solid = solid[((int64_t)ii * gm.ny + jj) * gm.nz + kk] > 0;  // SP:278
     word |= (1u | (inward_solid ? 2u : 0u)) << (2 * b);
   }
   face_bits[wi] = word;
@@ -859,34 +981,74 @@ __global__ void __launch_bounds__(kBlock) k_neighbors(const KP kp, const int32_t
   const uint32_t N = (uint32_t)kp.p.n_cells;
   if (x >= N) return;
   int i, j, k;
-  coords(kp, (uint32_t)cells[x], i, j, k);
+  coords(kp.geo, (uint32_t)cells[x], i, j, k);
   int32_t nb[18];
-  const Rows rb(rows, kp.n_row_words);
-  neighbor_ids(rb, kp, i, j, k, nb);
+  const Rows rb(rows, kp.geo.n_row_words);
+  neighbor_ids(rb, kp.geo, i, j, k, nb);
 #pragma unroll
   for (int q = 0; q < 18; ++q) nbr[(size_t)q * N + x] = nb[q];
 }
 
 __global__ void __launch_bounds__(kBlock) k_scatter(const KP kp, const int32_t *__restrict__ cells,
                                                     const float *__restrict__ rho, const float *__restrict__ v,
-                                                    int32_t ox, int32_t oy, int32_t oz, float *__restrict__ rho_d,
-                                                    float *__restrict__ v_d) {
+                                                    int64_t v_stride, int32_t ox, int32_t oy, int32_t oz,
+                                                    float *__restrict__ rho_d, float *__restrict__ v_d) {
   const uint32_t x = blockIdx.x * kBlock + threadIdx.x;
   const uint32_t N = (uint32_t)kp.p.n_cells;
   if (x >= N) return;
   int i, j, k;
-  coords(kp, (uint32_t)cells[x], i, j, k);
+  coords(kp.geo, (uint32_t)cells[x], i, j, k);
   const int64_t o = ((int64_t)i * oy + j) * oz + k;
   if (rho_d) rho_d[o] = rho[x];
   if (v_d) {
     v_d[3 * o] = v[x];
-    v_d[3 * o + 1] = v[(size_t)N + x];
-    v_d[3 * o + 2] = v[2 * (size_t)N + x];
+    v_d[3 * o + 1] = v[v_stride + x];
+    v_d[3 * o + 2] = v[2 * v_stride + x];
   }
 }
 
-// FP64 reductions: per-block partials, then one block folds them.
+// FP64 reductions: per-block partials, then one wave per quantity folds them
+// in a fixed order (the result does not depend on scheduling).
 constexpr int kRed = 8;
+
+__device__ __forceinline__ double fold(int r, double a, double b) {
+  return r == 4 ? (b > a ? b : a) : a + b;  // quantity 4 is a maximum, the others sums
+}
+
+// sums over a block: acc[r] of every thread -> out[r] (valid in thread 0..kRed-1)
+__device__ __forceinline__ void block_fold(double acc[kRed], double *__restrict__ out) {
+  __shared__ double s[kRed][kBlock / 64];
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+#pragma unroll
+  for (int r = 0; r < kRed; ++r) {
+    double val = acc[r];
+    for (int d = 32; d > 0; d >>= 1) val = fold(r, val, __shfl_down(val, d, 64));
+    if (lane == 0) s[r][wid] = val;
+  }
+  __syncthreads();
+  if (threadIdx.x < kRed) {
+    const int r = threadIdx.x;
+    double val = s[r][0];
+    for (int w = 1; w < kBlock / 64; ++w) val = fold(r, val, s[r][w]);
+    out[r] = val;
+  }
+}
+
+__device__ __forceinline__ bool accumulate(double acc[kRed], float r, float a, float b, float c) {
+  const bool fin = isfinite(r) && isfinite(a) && isfinite(b) && isfinite(c);
+  if (!fin) {
+    acc[5] += 1.0;
+    return false;
+  }
+  acc[0] += r;
+  acc[1] += a;
+  acc[2] += b;
+  acc[3] += c;
+  const double nrm = sqrt((double)a * a + (double)b * b + (double)c * c);
+  acc[4] = nrm > acc[4] ? nrm : acc[4];
+  return true;
+}
+
 __global__ void __launch_bounds__(kBlock) k_reduce_partial(const KP kp, const int32_t *__restrict__ cells,
                                                            const float *__restrict__ rho,
                                                            const float *__restrict__ v, int32_t i_lo,
@@ -894,54 +1056,55 @@ __global__ void __launch_bounds__(kBlock) k_reduce_partial(const KP kp, const in
   const uint32_t N = (uint32_t)kp.p.n_cells;
   double acc[kRed] = {0, 0, 0, 0, 0, 0, 0, 0};
   for (uint32_t x = blockIdx.x * kBlock + threadIdx.x; x < N; x += gridDim.x * kBlock) {
-    const float r = rho[x], a = v[x], b = v[(size_t)N + x], c = v[2 * (size_t)N + x];
-    const bool fin = isfinite(r) && isfinite(a) && isfinite(b) && isfinite(c);
-    if (!fin) {
-      acc[5] += 1.0;
-      continue;
-    }
-    acc[0] += r;
-    acc[1] += a;
-    acc[2] += b;
-    acc[3] += c;
-    const double nrm = sqrt((double)a * a + (double)b * b + (double)c * c);
-    acc[4] = nrm > acc[4] ? nrm : acc[4];
-    const int i = (int)fdiv((uint32_t)cells[x], kp.m_nyz, kp.s_nyz);
+    const float a = v[x];
+    if (!accumulate(acc, rho[x], a, v[(size_t)N + x], v[2 * (size_t)N + x])) continue;
+    const int i = (int)fdiv((uint32_t)cells[x], kp.geo.m_nyz, kp.geo.s_nyz);
     if (i >= i_lo && i <= i_hi) {
       acc[6] += a;
       acc[7] += 1.0;
     }
   }
-  __shared__ double s[kRed][kBlock / 64];
-  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-#pragma unroll
-  for (int r = 0; r < kRed; ++r) {
-    double val = acc[r];
-    for (int d = 32; d > 0; d >>= 1) {
-      const double o = __shfl_down(val, d, 64);
-      val = r == 4 ? (o > val ? o : val) : val + o;
-    }
-    if (lane == 0) s[r][wid] = val;
-  }
-  __syncthreads();
-  if (threadIdx.x < kRed) {
-    const int r = threadIdx.x;
-    double val = s[r][0];
-    for (int w = 1; w < kBlock / 64; ++w) val = r == 4 ? (s[r][w] > val ? s[r][w] : val) : val + s[r][w];
-    part[(size_t)blockIdx.x * kRed + r] = val;
-  }
+  double out[kRed];
+  block_fold(acc, out);
+  if (threadIdx.x < kRed) part[(size_t)blockIdx.x * kRed + threadIdx.x] = out[threadIdx.x];
 }
 
-__global__ void __launch_bounds__(64) k_reduce_final(const double *__restrict__ part, int nparts,
-                                                     double *__restrict__ out) {
-  if (threadIdx.x >= kRed) return;
-  const int r = threadIdx.x;
+// one wave per quantity r: lanes stride over the partials, then a shuffle tree
+__global__ void __launch_bounds__(64 * kRed) k_reduce_final(const double *__restrict__ part, int nparts,
+                                                           double *__restrict__ out) {
+  const int r = threadIdx.x >> 6, lane = threadIdx.x & 63;
   double val = 0.0;
-  for (int b = 0; b < nparts; ++b) {
-    const double o = part[(size_t)b * kRed + r];
-    val = r == 4 ? (o > val ? o : val) : val + o;
+  for (int b = lane; b < nparts; b += 64) val = fold(r, val, part[(size_t)b * kRed + r]);
+  for (int d = 32; d > 0; d >>= 1) val = fold(r, val, __shfl_down(val, d, 64));
+  if (lane == 0) out[r] = val;
+}
+
+// Per-sample reductions of a batch: one block per chunk, then one wave per
+// (sample, quantity) over the sample's chunks.  out[s][r]: sum rho, sum v (3),
+// max |v|, non-finite cells, fluid cells, 0.
+__global__ void __launch_bounds__(kBlock) k_reduce_batch_partial(const Batch bt, const float *__restrict__ rho,
+                                                                 const float *__restrict__ v, uint32_t N,
+                                                                 double *__restrict__ part) {
+  const uint32_t c = blockIdx.x;
+  const SampleDesc &d = bt.desc[bt.chunk_sample[c]];
+  double acc[kRed] = {0, 0, 0, 0, 0, 0, 0, 0};
+  for (uint32_t x = c * kChunk + threadIdx.x; x < (c + 1) * kChunk && x < d.cell_end; x += kBlock) {
+    accumulate(acc, rho[x], v[x], v[(size_t)N + x], v[2 * (size_t)N + x]);
+    acc[6] += 1.0;
   }
-  out[r] = val;
+  double out[kRed];
+  block_fold(acc, out);
+  if (threadIdx.x < kRed) part[(size_t)c * kRed + threadIdx.x] = out[threadIdx.x];
+}
+
+__global__ void __launch_bounds__(64 * kRed) k_reduce_batch_final(const Batch bt, const double *__restrict__ part,
+                                                                 double *__restrict__ out) {
+  const int r = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const SampleDesc &d = bt.desc[blockIdx.x];
+  double val = 0.0;
+  for (uint32_t c = d.chunk_begin + lane; c < d.chunk_end; c += 64) val = fold(r, val, part[(size_t)c * kRed + r]);
+  for (int dd = 32; dd > 0; dd >>= 1) val = fold(r, val, __shfl_down(val, dd, 64));
+  if (lane == 0) out[(size_t)blockIdx.x * kRed + r] = val;
 }
 
 constexpr int kRedBlocks = 1024;
@@ -1050,6 +1213,137 @@ int launch_init(hipStream_t s, const KP &kp, void *g, const int32_t *cells, void
   return check_launch("lbm3d_init");
 }
 
+// ---- batches ----------------------------------------------------------------
+constexpr size_t kDescAlign = 256;
+
+size_t desc_table_bytes(int32_t n) {
+  return ((size_t)n * sizeof(SampleDesc) + kDescAlign - 1) / kDescAlign * kDescAlign;
+}
+
+Batch batch_view(const void *desc, int32_t n_samples, const uint8_t *active) {
+  Batch b;
+  b.desc = (const SampleDesc *)desc;
+  b.chunk_sample = (const int32_t *)((const char *)desc + desc_table_bytes(n_samples));
+  b.active = active;
+  return b;
+}
+
+// compute units of the current device, cached per device
+int device_cus() {
+  static std::atomic<int> cache[64];
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0) return 256;
+  if (dev < 64) {
+    const int c = cache[dev].load(std::memory_order_relaxed);
+    if (c > 0) return c;
+  }
+  int cus = 0;
+  if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus < 1) cus = 256;
+  if (dev < 64) cache[dev].store(cus, std::memory_order_relaxed);
+  return cus;
+}
+
+constexpr lbm3d_launch kDefaultLaunch[2] = {{kBlock, 0, LBM3D_REVERSE_EVEN, 0}, {kBlock, 0, 0, 0}};
+
+bool valid_launch(const lbm3d_launch &c) {
+  return (c.block == 128 || c.block == 256 || c.block == 512) && c.grid_per_cu >= 0 && c.grid_per_cu <= 64;
+}
+
+template <typename T, bool ODD, bool FORCED>
+void launch_batch_parity(hipStream_t s, const KP &kp, const Batch &bt, void *g, const int32_t *cells,
+                         const uint4 *rows, const uint32_t *face_bits, void *face_u, int first,
+                         const lbm3d_launch &c) {
+  using R = typename St<T>::R;
+  const uint32_t n_tiles = (uint32_t)kp.p.n_cells / (uint32_t)c.block;
+  uint32_t grid = n_tiles;
+  if (c.grid_per_cu > 0) {
+    const uint64_t g_max = (uint64_t)device_cus() * (uint64_t)c.grid_per_cu;
+    grid = g_max < n_tiles ? (uint32_t)g_max : n_tiles;
+  }
+  const int rev = c.reverse ? 1 : 0;
+#define LBM3D_BATCH_LAUNCH(B)                                                                          \
+  k_step_batch<T, ODD, FORCED, B><<<grid, B, 0, s>>>(kp, bt, (T *)g, cells, rows, face_bits, (R *)face_u, first, \
+                                                     n_tiles, rev)
+  switch (c.block) {
+    case 128: LBM3D_BATCH_LAUNCH(128); break;
+    case 512: LBM3D_BATCH_LAUNCH(512); break;
+    default: LBM3D_BATCH_LAUNCH(256); break;
+  }
+#undef LBM3D_BATCH_LAUNCH
+}
+
+template <typename T>
+int launch_batch_step(hipStream_t s, const KP &kp, const Batch &bt, void *g, const int32_t *cells, const uint4 *rows,
+                      const uint32_t *face_bits, void *face_u, int64_t t, const lbm3d_launch *cfg) {
+  const lbm3d_launch &c = cfg ? cfg[t & 1] : kDefaultLaunch[t & 1];
+  const int first = t == 0;
+  if (kp.p.force_on) {
+    if (t & 1) launch_batch_parity<T, true, true>(s, kp, bt, g, cells, rows, face_bits, face_u, first, c);
+    else launch_batch_parity<T, false, true>(s, kp, bt, g, cells, rows, face_bits, face_u, first, c);
+  } else {
+    if (t & 1) launch_batch_parity<T, true, false>(s, kp, bt, g, cells, rows, face_bits, face_u, first, c);
+    else launch_batch_parity<T, false, false>(s, kp, bt, g, cells, rows, face_bits, face_u, first, c);
+  }
+  return check_launch("lbm3d_batch_step");
+}
+
+template <typename T>
+int launch_batch_macros(hipStream_t s, const KP &kp, const Batch &bt, const void *g, const int32_t *cells,
+                        const uint4 *rows, const uint32_t *face_bits, const void *face_u, int64_t t, void *rho,
+                        void *v) {
+  using R = typename St<T>::R;
+  const unsigned grid = (unsigned)((uint32_t)kp.p.n_cells / kBlock);
+  if (t & 1)
+    k_macros_batch<T, true><<<grid, kBlock, 0, s>>>(kp, bt, (const T *)g, cells, rows, face_bits, (const R *)face_u,
+                                                    t == 0, (R *)rho, (R *)v);
+  else
+    k_macros_batch<T, false><<<grid, kBlock, 0, s>>>(kp, bt, (const T *)g, cells, rows, face_bits, (const R *)face_u,
+                                                     t == 0, (R *)rho, (R *)v);
+  return check_launch("lbm3d_batch_macros");
+}
+
+template <typename T>
+int launch_batch_init(hipStream_t s, const KP &kp, const Batch &bt, void *g, const int32_t *cells, void *face_u,
+                      const float *rho0, const float *u0) {
+  using R = typename St<T>::R;
+  const unsigned grid = (unsigned)((uint32_t)kp.p.n_cells / kBlock);
+  k_init_batch<T><<<grid, kBlock, 0, s>>>(kp, bt, (T *)g, cells, (R *)face_u, rho0, u0);
+  return check_launch("lbm3d_batch_init");
+}
+
+// physical parameters and the batch-wide cell count of a batch call
+int check_batch(const lbm3d_params *p, int32_t n_samples, const void *desc) {
+  if (!p) return fail(LBM3D_EINVAL, "%s", "params is NULL");
+  if (n_samples < 1 || !desc) return fail(LBM3D_EINVAL, "%s", "a batch needs n_samples >= 1 and a descriptor");
+  if (p->n_cells <= 0 || p->n_cells % (int32_t)kChunk != 0)
+    return fail(LBM3D_EINVAL, "%s", "batch n_cells must be the layout's cell count (a positive multiple of "
+                                    "LBM3D_BATCH_ALIGN)");
+  if (p->n_cells / (int32_t)kChunk < n_samples)
+    return fail(LBM3D_EINVAL, "%s", "batch n_cells is smaller than one chunk per sample");
+  return LBM3D_OK;
+}
+
+int check_batch_solver(const lbm3d_params *p, int dtype, int32_t n_samples, const void *desc, const void *g,
+                       const int32_t *cells, const void *rows, const uint32_t *face_bits) {
+  int rc = check_batch(p, n_samples, desc);
+  if (rc) return rc;
+  if (!valid_dtype(dtype)) return fail(LBM3D_EINVAL, "%s", "unknown dtype");
+  if ((rc = check_dtype_size(p, dtype))) return rc;
+  if (!g || !cells || !rows || !face_bits)
+    return fail(LBM3D_EINVAL, "%s", "populations, cells, rows or face_bits is NULL");
+  return LBM3D_OK;
+}
+
+KP make_kp_batch(const lbm3d_params &p) {
+  KP k;
+  k.p = p;
+  fill_phys(k.pf, p);
+  fill_phys(k.pd, p);
+  k.S = slot_stride((uint32_t)p.n_cells);
+  memset(&k.geo, 0, sizeof(k.geo));  // every sample's geometry comes from its descriptor
+  return k;
+}
+
 }  // namespace
 
 extern "C" {
@@ -1109,16 +1403,24 @@ int lbm3d_compact_fill(void *stream, const lbm3d_params *p, const int8_t *solid,
   return check_launch("lbm3d_compact_fill");
 }
 
-int lbm3d_build_rows(void *stream, const lbm3d_params *p, const int8_t *solid, const int32_t *d2c,
-                     lbm3d_row_word *rows) {
+int lbm3d_build_rows_at(void *stream, const lbm3d_params *p, const int8_t *solid, const int32_t *d2c,
+                        int32_t id_offset, lbm3d_row_word *rows) {
   int rc = check_params(p);
   if (rc) return rc;
   if (!solid || !d2c || !rows) return fail(LBM3D_EINVAL, "%s", "NULL argument to lbm3d_build_rows");
-  const KP kp = make_kp(*p);
+  if (id_offset < 0) return fail(LBM3D_EINVAL, "%s", "negative id offset");
+  if ((int64_t)id_offset + (int64_t)p->nx * p->ny * p->nz > INT32_MAX)
+    return fail(LBM3D_ESIZE, "%s", "id offset plus cells exceeds int32");
+  const Geo gm = make_geo(p->nx, p->ny, p->nz, p->face_u_off);
   const int64_t n = lbm3d_row_words(p);
-  k_build_rows<<<(unsigned)((n + kBlock - 1) / kBlock), kBlock, 0, (hipStream_t)stream>>>(kp, solid, d2c,
+  k_build_rows<<<(unsigned)((n + kBlock - 1) / kBlock), kBlock, 0, (hipStream_t)stream>>>(gm, solid, d2c, id_offset,
                                                                                          (uint4 *)rows);
   return check_launch("lbm3d_build_rows");
+}
+
+int lbm3d_build_rows(void *stream, const lbm3d_params *p, const int8_t *solid, const int32_t *d2c,
+                     lbm3d_row_word *rows) {
+  return lbm3d_build_rows_at(stream, p, solid, d2c, 0, rows);
 }
 
 int lbm3d_build_face_bits(void *stream, const lbm3d_params *p, const int8_t *solid, const int32_t *cells,
@@ -1224,17 +1526,20 @@ int lbm3d_macros(void *stream, const lbm3d_params *p, int dtype, const void *g, 
 }
 
 int lbm3d_scatter_dense(void *stream, const lbm3d_params *p, const int32_t *cells, const float *rho,
-                        const float *v, int32_t ox, int32_t oy, int32_t oz, float *rho_dense, float *v_dense) {
+                        const float *v, int64_t v_stride, int32_t ox, int32_t oy, int32_t oz, float *rho_dense,
+                        float *v_dense) {
   int rc = check_params(p);
   if (rc) return rc;
   if ((rc = check_cells(p))) return rc;
   if (ox < p->nx || oy < p->ny || oz < p->nz) return fail(LBM3D_EINVAL, "%s", "dense extent smaller than the grid");
+  if (v_stride == 0) v_stride = p->n_cells;
+  if (v_stride < p->n_cells) return fail(LBM3D_EINVAL, "%s", "v_stride smaller than n_cells");
   if (p->n_cells == 0) return LBM3D_OK;
   if (!cells || (rho_dense && !rho) || (v_dense && !v))
     return fail(LBM3D_EINVAL, "%s", "NULL argument to lbm3d_scatter_dense");
   const KP kp = make_kp(*p);
   const unsigned grid = (unsigned)((p->n_cells + kBlock - 1) / kBlock);
-  k_scatter<<<grid, kBlock, 0, (hipStream_t)stream>>>(kp, cells, rho, v, ox, oy, oz, rho_dense, v_dense);
+  k_scatter<<<grid, kBlock, 0, (hipStream_t)stream>>>(kp, cells, rho, v, v_stride, ox, oy, oz, rho_dense, v_dense);
   return check_launch("lbm3d_scatter_dense");
 }
 
@@ -1258,8 +1563,179 @@ int lbm3d_reduce(void *stream, const lbm3d_params *p, const int32_t *cells, cons
   hipStream_t s = (hipStream_t)stream;
   k_reduce_partial<<<blocks, kBlock, 0, s>>>(kp, cells, rho, v, i_lo, i_hi, (double *)workspace);
   if ((rc = check_launch("lbm3d_reduce"))) return rc;
-  k_reduce_final<<<1, 64, 0, s>>>((const double *)workspace, blocks, out);
+  k_reduce_final<<<1, 64 * kRed, 0, s>>>((const double *)workspace, blocks, out);
   return check_launch("lbm3d_reduce/final");
+}
+
+
+// ---- batches ---------------------------------------------------------------
+int lbm3d_batch_layout(const lbm3d_params *p, lbm3d_sample *samples, int32_t n_samples, lbm3d_batch_sizes *sizes) {
+  if (!p || !samples || !sizes || n_samples < 1) return fail(LBM3D_EINVAL, "%s", "NULL argument or no samples");
+  int64_t cell = 0, rows = 0, fu = 0;
+  for (int32_t s = 0; s < n_samples; ++s) {
+    lbm3d_sample &sm = samples[s];
+    lbm3d_params q = *p;
+    q.nx = sm.nx;
+    q.ny = sm.ny;
+    q.nz = sm.nz;
+    q.n_cells = sm.n_cells;
+    int rc = check_params(&q);
+    if (rc) return rc;
+    if ((rc = check_cells(&q))) return rc;
+    sm.cell_begin = (int32_t)cell;
+    sm.rows_begin = rows;
+    sm.face_u_begin = (int32_t)fu;
+    // every sample owns at least one chunk, so the chunk map names each
+    const int64_t span = sm.n_cells > 0 ? sm.n_cells : 1;
+    cell += (span + kChunk - 1) / kChunk * kChunk;
+    rows += lbm3d_row_words(&q);
+    for (int f = 0; f < 6; ++f)
+      if (p->bc_mode[f] == 1) fu += 3 * lbm3d_face_plane(&q, f);
+    if (cell > INT32_MAX - (int64_t)kChunk || fu > INT32_MAX || rows > UINT32_MAX)
+      return fail(LBM3D_ESIZE, "%s", "batch exceeds 2^31 cells or face nodes");
+  }
+  sizes->cells = cell;
+  sizes->row_words = rows;
+  sizes->face_u = fu;
+  sizes->desc_bytes = (int64_t)desc_table_bytes(n_samples) + (cell / kChunk) * (int64_t)sizeof(int32_t);
+  return LBM3D_OK;
+}
+
+int lbm3d_batch_desc(const lbm3d_params *p, const lbm3d_sample *samples, int32_t n_samples, void *host_desc,
+                     size_t bytes) {
+  if (!p || !samples || !host_desc || n_samples < 1) return fail(LBM3D_EINVAL, "%s", "NULL argument or no samples");
+  const lbm3d_sample &last = samples[n_samples - 1];
+  const int64_t span = last.n_cells > 0 ? last.n_cells : 1;
+  const int64_t cells = last.cell_begin + (span + kChunk - 1) / kChunk * kChunk;
+  const int64_t n_chunks = cells / kChunk;
+  if (bytes < desc_table_bytes(n_samples) + (size_t)n_chunks * sizeof(int32_t))
+    return fail(LBM3D_EINVAL, "%s", "descriptor buffer too small");
+  memset(host_desc, 0, desc_table_bytes(n_samples));
+  SampleDesc *d = (SampleDesc *)host_desc;
+  int32_t *map = (int32_t *)((char *)host_desc + desc_table_bytes(n_samples));
+  int64_t expect = 0;
+  for (int32_t s = 0; s < n_samples; ++s) {
+    const lbm3d_sample &sm = samples[s];
+    if (sm.cell_begin != expect) return fail(LBM3D_EINVAL, "%s", "samples not placed by lbm3d_batch_layout");
+    int32_t off[6];
+    int64_t fu = sm.face_u_begin;
+    lbm3d_params q = *p;
+    q.nx = sm.nx;
+    q.ny = sm.ny;
+    q.nz = sm.nz;
+    q.n_cells = sm.n_cells;
+    int rc = check_params(&q);
+    if (rc) return rc;
+    if ((rc = check_cells(&q))) return rc;
+    for (int f = 0; f < 6; ++f) {
+      off[f] = p->bc_mode[f] == 1 ? (int32_t)fu : -1;
+      if (p->bc_mode[f] == 1) fu += 3 * lbm3d_face_plane(&q, f);
+    }
+    d[s].geo = make_geo(sm.nx, sm.ny, sm.nz, off);
+    d[s].cell_begin = (uint32_t)sm.cell_begin;
+    d[s].cell_end = (uint32_t)(sm.cell_begin + (sm.n_cells > 0 ? sm.n_cells : 0));
+    const int64_t sp = sm.n_cells > 0 ? sm.n_cells : 1;
+    const int64_t c0 = sm.cell_begin / kChunk, c1 = c0 + (sp + kChunk - 1) / kChunk;
+    d[s].chunk_begin = (uint32_t)c0;
+    d[s].chunk_end = (uint32_t)c1;
+    d[s].rows_begin = (uint64_t)sm.rows_begin;
+    for (int64_t c = c0; c < c1; ++c) map[c] = s;
+    expect = c1 * kChunk;
+  }
+  return LBM3D_OK;
+}
+
+int lbm3d_batch_init(void *stream, const lbm3d_params *p, int dtype, void *g, const void *desc, int32_t n_samples,
+                     const int32_t *cells, void *face_u, const float *rho0, const float *u0) {
+  int rc = check_batch(p, n_samples, desc);
+  if (rc) return rc;
+  if (!valid_dtype(dtype)) return fail(LBM3D_EINVAL, "%s", "unknown dtype");
+  if ((rc = check_dtype_size(p, dtype))) return rc;
+  if (!g || !cells) return fail(LBM3D_EINVAL, "%s", "populations or cells are NULL");
+  for (int f = 0; f < 6; ++f)
+    if (p->bc_mode[f] == 1 && !face_u) return fail(LBM3D_EINVAL, "%s", "face_u is NULL with a pressure face");
+  const KP kp = make_kp_batch(*p);
+  const Batch bt = batch_view(desc, n_samples, nullptr);
+  hipStream_t s = (hipStream_t)stream;
+  switch (dtype) {
+    case LBM3D_F32: return launch_batch_init<float>(s, kp, bt, g, cells, face_u, rho0, u0);
+    case LBM3D_F16: return launch_batch_init<__half>(s, kp, bt, g, cells, face_u, rho0, u0);
+    case LBM3D_F64: return launch_batch_init<double>(s, kp, bt, g, cells, face_u, rho0, u0);
+    default: return launch_batch_init<__hip_bfloat16>(s, kp, bt, g, cells, face_u, rho0, u0);
+  }
+}
+
+int lbm3d_batch_run(void *stream, const lbm3d_params *p, int dtype, void *g, const void *desc, int32_t n_samples,
+                    const int32_t *cells, const lbm3d_row_word *rows, const uint32_t *face_bits, void *face_u,
+                    const uint8_t *active, int64_t t0, int32_t nsteps, const lbm3d_launch *cfg) {
+  int rc = check_batch_solver(p, dtype, n_samples, desc, g, cells, rows, face_bits);
+  if (rc) return rc;
+  if (t0 < 0 || nsteps < 0) return fail(LBM3D_EINVAL, "%s", "negative step index or count");
+  if (cfg && (!valid_launch(cfg[0]) || !valid_launch(cfg[1])))
+    return fail(LBM3D_EINVAL, "%s", "launch shape: block must be 128, 256 or 512 and grid_per_cu 0..64");
+  for (int f = 0; f < 6; ++f)
+    if (p->bc_mode[f] == 1 && !face_u) return fail(LBM3D_EINVAL, "%s", "face_u is NULL with a pressure face");
+  const KP kp = make_kp_batch(*p);
+  const Batch bt = batch_view(desc, n_samples, active);
+  hipStream_t s = (hipStream_t)stream;
+  const uint4 *r = (const uint4 *)rows;
+  for (int32_t k = 0; k < nsteps; ++k) {
+    const int64_t t = t0 + k;
+    switch (dtype) {
+      case LBM3D_F32: rc = launch_batch_step<float>(s, kp, bt, g, cells, r, face_bits, face_u, t, cfg); break;
+      case LBM3D_F16: rc = launch_batch_step<__half>(s, kp, bt, g, cells, r, face_bits, face_u, t, cfg); break;
+      case LBM3D_F64: rc = launch_batch_step<double>(s, kp, bt, g, cells, r, face_bits, face_u, t, cfg); break;
+      default: rc = launch_batch_step<__hip_bfloat16>(s, kp, bt, g, cells, r, face_bits, face_u, t, cfg); break;
+    }
+    if (rc) return rc;
+  }
+  return LBM3D_OK;
+}
+
+int lbm3d_batch_step(void *stream, const lbm3d_params *p, int dtype, void *g, const void *desc, int32_t n_samples,
+                     const int32_t *cells, const lbm3d_row_word *rows, const uint32_t *face_bits, void *face_u,
+                     const uint8_t *active, int64_t t, const lbm3d_launch *cfg) {
+  return lbm3d_batch_run(stream, p, dtype, g, desc, n_samples, cells, rows, face_bits, face_u, active, t, 1, cfg);
+}
+
+int lbm3d_batch_macros(void *stream, const lbm3d_params *p, int dtype, const void *g, const void *desc,
+                       int32_t n_samples, const int32_t *cells, const lbm3d_row_word *rows,
+                       const uint32_t *face_bits, const void *face_u, int64_t t, void *rho, void *v) {
+  int rc = check_batch_solver(p, dtype, n_samples, desc, g, cells, rows, face_bits);
+  if (rc) return rc;
+  if (t < 0) return fail(LBM3D_EINVAL, "%s", "negative step index");
+  if (!rho || !v) return fail(LBM3D_EINVAL, "%s", "rho or v output is NULL");
+  const KP kp = make_kp_batch(*p);
+  const Batch bt = batch_view(desc, n_samples, nullptr);
+  hipStream_t s = (hipStream_t)stream;
+  const uint4 *r = (const uint4 *)rows;
+  switch (dtype) {
+    case LBM3D_F32: return launch_batch_macros<float>(s, kp, bt, g, cells, r, face_bits, face_u, t, rho, v);
+    case LBM3D_F16: return launch_batch_macros<__half>(s, kp, bt, g, cells, r, face_bits, face_u, t, rho, v);
+    case LBM3D_F64: return launch_batch_macros<double>(s, kp, bt, g, cells, r, face_bits, face_u, t, rho, v);
+    default: return launch_batch_macros<__hip_bfloat16>(s, kp, bt, g, cells, r, face_bits, face_u, t, rho, v);
+  }
+}
+
+size_t lbm3d_batch_reduce_workspace_bytes(int64_t cells) {
+  const int64_t chunks = cells > 0 ? (cells + kChunk - 1) / kChunk : 1;
+  return (size_t)chunks * kRed * sizeof(double);
+}
+
+int lbm3d_batch_reduce(void *stream, const lbm3d_params *p, const void *desc, int32_t n_samples, const float *rho,
+                       const float *v, double *out, void *workspace, size_t workspace_bytes) {
+  int rc = check_batch(p, n_samples, desc);
+  if (rc) return rc;
+  if (!rho || !v || !out || !workspace) return fail(LBM3D_EINVAL, "%s", "NULL argument to lbm3d_batch_reduce");
+  if (workspace_bytes < lbm3d_batch_reduce_workspace_bytes(p->n_cells))
+    return fail(LBM3D_EINVAL, "%s", "reduction workspace too small");
+  const Batch bt = batch_view(desc, n_samples, nullptr);
+  hipStream_t s = (hipStream_t)stream;
+  const uint32_t chunks = (uint32_t)p->n_cells / kChunk;
+  k_reduce_batch_partial<<<chunks, kBlock, 0, s>>>(bt, rho, v, (uint32_t)p->n_cells, (double *)workspace);
+  if ((rc = check_launch("lbm3d_batch_reduce"))) return rc;
+  k_reduce_batch_final<<<(unsigned)n_samples, 64 * kRed, 0, s>>>(bt, (const double *)workspace, out);
+  return check_launch("lbm3d_batch_reduce/final");
 }
 
 }  // extern "C"

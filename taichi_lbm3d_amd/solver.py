@@ -499,7 +499,7 @@ class LB3D_Solver_Single_Phase:
         v_d = torch.zeros(shape + (3,), dtype=torch.float32, device=dev)
         with torch.cuda.device(dev):
             _lib.lib().call("lbm3d_scatter_dense", self._stream(), _ref(p), _ptr(self._dev["cells"]),
-                            _ptr(rho_c), _ptr(v_c), shape[0], shape[1], shape[2], _ptr(rho_d), _ptr(v_d))
+                            _ptr(rho_c), _ptr(v_c), 0, shape[0], shape[1], shape[2], _ptr(rho_d), _ptr(v_d))
         return rho_d.cpu().numpy(), v_d.cpu().numpy()
 
     def reduce(self, i_lo: int = 0, i_hi: Optional[int] = None):

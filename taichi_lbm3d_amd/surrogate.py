@@ -1,0 +1,274 @@
+"""Warm-start surrogate: a 3-D U-Net that predicts steady (v, rho - 1) from the
+geometry, on the hand-written BF16 MFMA convolutions of conv3d.hip.
+
+No reference counterpart: the reference always starts at rest (init,
+SP:160-170).  The network's prediction initialises f at equilibrium through
+init_from_macros (a warm start); a cold start stays the fallback.
+
+Network: 4 levels of 16-32-64-128 channels, two 3x3x3 convolutions with ReLU
+per level, 2x2x2 max pooling down, nearest-neighbour upsampling and channel
+concatenation up, a 1x1x1 head.  Activations and weights are BF16 in the
+kernels, accumulation FP32; master weights and Adam state are FP32.  Tensors
+are channels-last (N, X, Y, Z, C) throughout.
+
+Inputs (2 channels): solid mask, Euclidean distance to the nearest solid cell
+/ DIST_SCALE.  Outputs (4 channels): (vx, vy, vz) / (fx / nu_eff) and
+(rho - 1) / (fx * RHO_SCALE): fields in units of the body force, so one
+network serves any small force.
+"""
+from __future__ import annotations
+
+import ctypes
+import math
+
+import numpy as np
+
+from . import _lib
+
+CIN_PAD = 16        # input channels are zero-padded to the kernels' multiple of 16
+HEAD_PAD = 16       # the 4 output channels are computed as 16 (12 zero rows)
+DIST_SCALE = 8.0
+RHO_SCALE = 100.0
+CHANNELS = (16, 32, 64, 128)
+
+
+def _torch():
+    import torch
+    return torch
+
+
+def _vp(t):
+    return ctypes.c_void_p(t.data_ptr())
+
+
+def _stream(dev):
+    torch = _torch()
+    return ctypes.c_void_p(torch.cuda.current_stream(dev).cuda_stream)
+
+
+# ---- the convolution as an autograd function ----------------------------------------------
+
+def conv_fwd(x, w_bf16, bias, cout, ks, relu):
+    """x (N,X,Y,Z,Cin) bf16 contiguous, w_bf16 (cout, kpad) bf16 -> (N,X,Y,Z,cout) bf16."""
+    torch = _torch()
+    n, d, h, wd, cin = x.shape
+    y = torch.empty((n, d, h, wd, cout), dtype=torch.bfloat16, device=x.device)
+    _lib.lib().call("lbm3d_conv3d_fwd", _stream(x.device), _vp(x), _vp(w_bf16),
+                    _vp(bias) if bias is not None else None, _vp(y), n, d, h, wd, cin, cout, ks, int(relu))
+    return y
+
+
+def conv_dgrad(dy, w_bf16, cin, ks):
+    torch = _torch()
+    n, d, h, wd, cout = dy.shape
+    lib = _lib.lib()
+    kpad_t = lib.cdll.lbm3d_conv_kpad(cout, ks)
+    wt = torch.empty((cin, kpad_t), dtype=torch.bfloat16, device=dy.device)
+    s = _stream(dy.device)
+    lib.call("lbm3d_conv3d_wflip", s, _vp(w_bf16), _vp(wt), cin, cout, ks)
+    dx = torch.empty((n, d, h, wd, cin), dtype=torch.bfloat16, device=dy.device)
+    lib.call("lbm3d_conv3d_dgrad", s, _vp(dy), _vp(wt), _vp(dx), n, d, h, wd, cin, cout, ks)
+    return dx
+
+
+def conv_wgrad(x, dy, ks):
+    torch = _torch()
+    n, d, h, wd, cin = x.shape
+    cout = dy.shape[-1]
+    lib = _lib.lib()
+    kpad = lib.cdll.lbm3d_conv_kpad(cin, ks)
+    ws_bytes = lib.size("lbm3d_conv3d_wgrad_workspace_bytes", n, d, h, wd, cin, cout, ks)
+    ws = torch.empty(ws_bytes, dtype=torch.uint8, device=x.device)
+    dw = torch.empty((cout, kpad), dtype=torch.float32, device=x.device)
+    lib.call("lbm3d_conv3d_wgrad", _stream(x.device), _vp(x), _vp(dy), _vp(dw), n, d, h, wd, cin, cout, ks,
+             _vp(ws), ws_bytes)
+    return dw
+
+
+def _make_conv_function():
+    torch = _torch()
+
+    class Conv3dMFMA(torch.autograd.Function):
+        """y = act(conv3d(x, w) + b) with the MFMA kernels in both directions."""
+
+        @staticmethod
+        def forward(ctx, x, w, b, ks, relu, need_dx):
+            x = x.contiguous()
+            wb = w.to(torch.bfloat16).contiguous()
+            y = conv_fwd(x, wb, b, w.shape[0], ks, relu)
+            ctx.save_for_backward(x, wb, y if relu else None)
+            ctx.ks, ctx.relu, ctx.need_dx = ks, relu, need_dx
+            return y
+
+        @staticmethod
+        def backward(ctx, gy):
+            x, wb, y = ctx.saved_tensors
+            g = gy.contiguous().to(torch.bfloat16)
+            if ctx.relu:
+                g = (g * (y > 0)).contiguous()
+            dw = conv_wgrad(x, g, ctx.ks)
+            db = g.float().sum(dim=(0, 1, 2, 3))
+            dx = conv_dgrad(g, wb, x.shape[-1], ctx.ks) if ctx.need_dx else None
+            return dx, dw, db, None, None, None
+
+    return Conv3dMFMA
+
+
+_CONV = None
+
+
+def conv3d(x, w, b, ks: int = 3, relu: bool = True, need_dx: bool = True):
+    global _CONV
+    if _CONV is None:
+        _CONV = _make_conv_function()
+    return _CONV.apply(x, w, b, ks, relu, need_dx)
+
+
+# ---- the network ------------------------------------------------------------------------------
+
+def _pool(x):
+    n, d, h, w, c = x.shape
+    return x.view(n, d // 2, 2, h // 2, 2, w // 2, 2, c).amax(dim=(2, 4, 6))
+
+
+def _up(x):
+    n, d, h, w, c = x.shape
+    return x[:, :, None, :, None, :, None, :].expand(n, d, 2, h, 2, w, 2, c).reshape(n, 2 * d, 2 * h, 2 * w, c)
+
+
+class UNet3D:
+    """Parameters as FP32 master tensors; forward through the MFMA convolutions."""
+
+    def __init__(self, device, seed: int = 0, channels=CHANNELS):
+        torch = _torch()
+        self.device = device
+        self.channels = tuple(channels)
+        c1, c2, c3, c4 = self.channels
+        specs = [("e1a", CIN_PAD, c1, 3), ("e1b", c1, c1, 3),
+                 ("e2a", c1, c2, 3), ("e2b", c2, c2, 3),
+                 ("e3a", c2, c3, 3), ("e3b", c3, c3, 3),
+                 ("e4a", c3, c4, 3), ("e4b", c4, c4, 3),
+                 ("d3a", c4 + c3, c3, 3), ("d3b", c3, c3, 3),
+                 ("d2a", c3 + c2, c2, 3), ("d2b", c2, c2, 3),
+                 ("d1a", c2 + c1, c1, 3), ("d1b", c1, c1, 3),
+                 ("head", c1, HEAD_PAD, 1)]
+        self.specs = {name: (cin, cout, ks) for name, cin, cout, ks in specs}
+        gen = torch.Generator().manual_seed(seed)
+        self.params = {}
+        lib = _lib.lib()
+        for name, cin, cout, ks in specs:
+            kpad = lib.cdll.lbm3d_conv_kpad(cin, ks)
+            k = ks ** 3 * cin
+            fan_in = k if name != "e1a" else ks ** 3 * 2
+            std = math.sqrt(2.0 / fan_in) if name != "head" else math.sqrt(1.0 / fan_in)
+            core = torch.randn(cout, ks ** 3, cin, generator=gen) * std
+            if name == "e1a":  # only the 2 real input channels carry weights
+                core[:, :, 2:] = 0
+            w = torch.zeros(cout, kpad)
+            w[:, :k] = core.reshape(cout, k)
+            if name == "head":
+                w[4:] = 0
+            self.params[name + ".w"] = w.to(device).requires_grad_(True)
+            self.params[name + ".b"] = torch.zeros(cout, device=device).requires_grad_(True)
+        # weights that must stay zero: input channels 2..15 of the first layer
+        cin, cout, ks = self.specs["e1a"]
+        kpad = lib.cdll.lbm3d_conv_kpad(cin, ks)
+        keep = torch.zeros(cout, kpad, device=device)
+        keep[:, : ks ** 3 * cin] = (torch.arange(ks ** 3 * cin, device=device) % cin < 2).float()
+        self._e1a_keep = keep
+
+    def parameters(self):
+        return list(self.params.values())
+
+    def _c(self, name, x, relu=True, need_dx=True):
+        cin, cout, ks = self.specs[name]
+        return conv3d(x, self.params[name + ".w"], self.params[name + ".b"], ks, relu, need_dx)
+
+    def forward(self, x):
+        """x (N,X,Y,Z,16) bf16 with channels 0-1 the inputs -> (N,X,Y,Z,4) float32.
+        X, Y, Z must be multiples of 8."""
+        torch = _torch()
+        e1 = self._c("e1b", self._c("e1a", x, need_dx=False))
+        e2 = self._c("e2b", self._c("e2a", _pool(e1)))
+        e3 = self._c("e3b", self._c("e3a", _pool(e2)))
+        e4 = self._c("e4b", self._c("e4a", _pool(e3)))
+        d3 = self._c("d3b", self._c("d3a", torch.cat([_up(e4), e3], dim=-1)))
+        d2 = self._c("d2b", self._c("d2a", torch.cat([_up(d3), e2], dim=-1)))
+        d1 = self._c("d1b", self._c("d1a", torch.cat([_up(d2), e1], dim=-1)))
+        out = self._c("head", d1, relu=False)
+        return out[..., :4].float()
+
+    __call__ = forward
+
+    def zero_head_padding(self):
+        """Keep the unused head rows and input-channel weights exactly zero."""
+        torch = _torch()
+        with torch.no_grad():
+            self.params["head.w"][4:] = 0
+            self.params["head.b"][4:] = 0
+            self.params["e1a.w"].mul_(self._e1a_keep)
+
+    def state_dict(self):
+        return {k: v.detach().cpu() for k, v in self.params.items()}
+
+    def load_state_dict(self, sd):
+        torch = _torch()
+        with torch.no_grad():
+            for k, v in sd.items():
+                self.params[k].copy_(v.to(self.device))
+
+
+# ---- inputs and targets -----------------------------------------------------------------------------
+
+def periodic_distance(solid: np.ndarray, margin: int = 24) -> np.ndarray:
+    """Euclidean distance of every cell to the nearest solid cell under periodic
+    wrap (exact up to `margin` cells; larger distances are capped there)."""
+    from scipy import ndimage
+    s = np.asarray(solid) != 0
+    if not s.any():
+        return np.full(s.shape, float(margin), np.float32)
+    m = min(margin, *s.shape)
+    padded = np.pad(~s, m, mode="wrap")
+    d = ndimage.distance_transform_edt(padded)
+    d = d[m:-m, m:-m, m:-m]
+    return np.minimum(d, margin).astype(np.float32)
+
+
+def make_inputs(solid: np.ndarray):
+    """(X, Y, Z, 2) float32: solid mask and distance / DIST_SCALE."""
+    s = (np.asarray(solid) != 0).astype(np.float32)
+    return np.stack([s, periodic_distance(solid) / DIST_SCALE], axis=-1)
+
+
+def make_targets(rho, v, solid, fx, nu_eff):
+    """(X, Y, Z, 4) float32 normalised targets; zero on solid cells."""
+    fluid = (np.asarray(solid) == 0)[..., None]
+    vt = np.asarray(v, np.float32) * (nu_eff / fx)
+    rt = (np.asarray(rho, np.float32) - 1.0)[..., None] / (fx * RHO_SCALE)
+    return np.where(fluid, np.concatenate([vt, rt], axis=-1), 0.0).astype(np.float32)
+
+
+def fields_from_prediction(pred, solid, fx, nu_eff):
+    """Inverse of make_targets: (rho, v) dense arrays from a (X,Y,Z,4) prediction."""
+    p = np.asarray(pred, np.float32)
+    fluid = (np.asarray(solid) == 0)[..., None]
+    v = np.where(fluid, p[..., :3] * (fx / nu_eff), 0.0).astype(np.float32)
+    rho = (1.0 + p[..., 3] * fx * RHO_SCALE).astype(np.float32)
+    return rho, v
+
+
+def to_network_input(inp, device):
+    """(X,Y,Z,2) float32 -> (1,X,Y,Z,16) bf16 on the device (channels 2..15 zero)."""
+    torch = _torch()
+    t = torch.zeros((1,) + inp.shape[:3] + (CIN_PAD,), dtype=torch.bfloat16, device=device)
+    t[..., :2] = torch.from_numpy(np.ascontiguousarray(inp)).to(device).to(torch.bfloat16)
+    return t
+
+
+def predict(net, solid, fx, nu_eff):
+    """Steady (rho, v) predicted for one periodic geometry (extents multiples of 8)."""
+    torch = _torch()
+    with torch.no_grad():
+        x = to_network_input(make_inputs(solid), net.device)
+        out = net(x)[0].cpu().numpy()
+    return fields_from_prediction(out, solid, fx, nu_eff)

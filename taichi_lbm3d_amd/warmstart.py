@@ -1,0 +1,249 @@
+"""Learned warm start, end to end on one GPU.
+
+    python taichi_lbm3d_amd/warmstart.py [--train-samples 64] [--iters 1000] [--batch 2]
+                                          [--crop 128] [--eval-samples 32] [--eval-size 192]
+                                          [--save-model PATH] [--out PATH]
+
+1. Training data: the batched workload's 64 seeded periodic sphere packs
+   (edges 48-160, porosity 0.25-0.45, radii 4-12, fx = 1e-5, niu = 1/6) run to
+   steady state by the batched solver; their converged fields are the targets.
+2. Training: the 3-D U-Net of surrogate.py with Adam (lr 1e-3) on random
+   periodic 128^3 crops of those samples (a periodic sample tiles space, so a
+   crop may wrap; y/z flips and the y-z swap are applied at random), masked MSE
+   on fluid cells, `iters` steps of `batch` crops.
+3. Evaluation: held-out seeded 192^3 packs (porosity 0.3, radii 4-12).  Cold
+   start (rest state) and warm start (f at the equilibrium of the predicted
+   fields) each run to the 1e-6 steady criterion in one batched launch;
+   reported: steps, wall-clock (the warm side includes inference), the
+   permeability error of the warm result against the cold one, the steps and
+   time until each sample's permeability is within 1 % of its cold-start value,
+   and the samples that fell back to a cold start (a non-finite prediction or
+   a run that does not converge).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from concurrent.futures import ProcessPoolExecutor
+
+import numpy as np
+
+if __package__ in (None, ""):  # run as a file
+    sys.path.insert(0, os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+
+from taichi_lbm3d_amd import surrogate as sg  # noqa: E402
+from taichi_lbm3d_amd.batch_bench import FX, NIU, make_samples  # noqa: E402
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def _inputs(solid):
+    return sg.make_inputs(solid)
+
+
+def converge(solids, graph_steps=100, max_steps=50_000, tol=1e-6, init_fields=None, track=None):
+    """Run a batch to steady state; returns (batch, steps, converged, wall_s,
+    history of (t, wall_s, permeability per sample))."""
+    import torch
+    from taichi_lbm3d_amd.batch import BatchSolver
+    b = BatchSolver(solids, niu=NIU, force=(FX, 0.0, 0.0))
+    if init_fields is not None:
+        b.init_from_macros(init_fields)
+    hist = []
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+
+    def cb(t, r):
+        hist.append((t, time.perf_counter() - t0, b.permeability(r)))
+
+    steps, conv = b.run_to_steady(tol=tol, interval=500, max_steps=max_steps, graph_steps=graph_steps, callback=cb)
+    torch.cuda.synchronize()
+    return b, steps, conv, time.perf_counter() - t0, hist
+
+
+def within(hist, k_ref, rel=0.01):
+    """Per sample: first (step, wall_s) from which |k - k_ref| / k_ref stays below rel."""
+    n = len(k_ref)
+    steps = np.full(n, -1, np.int64)
+    walls = np.full(n, np.nan)
+    for s in range(n):
+        for idx in range(len(hist)):
+            if all(abs(h[2][s] - k_ref[s]) <= rel * abs(k_ref[s]) for h in hist[idx:]):
+                steps[s], walls[s] = hist[idx][0], hist[idx][1]
+                break
+    return steps, walls
+
+
+class CropSampler:
+    """Random periodic crops of converged samples, on the GPU."""
+
+    def __init__(self, inputs, targets, solids, device, seed=0):
+        import torch
+        self.torch = torch
+        self.inp = [torch.from_numpy(i).to(device) for i in inputs]
+        self.tgt = [torch.from_numpy(t).to(device) for t in targets]
+        self.fluid = [torch.from_numpy((s == 0).astype(np.float32)).to(device) for s in solids]
+        self.device = device
+        self.rng = np.random.default_rng(seed)
+
+    def crop(self, size):
+        torch = self.torch
+        s = int(self.rng.integers(len(self.inp)))
+        L = self.inp[s].shape[:3]
+        idx = [((torch.arange(size, device=self.device) + int(self.rng.integers(L[a]))) % L[a]) for a in range(3)]
+        x = self.inp[s][idx[0]][:, idx[1]][:, :, idx[2]]
+        y = self.tgt[s][idx[0]][:, idx[1]][:, :, idx[2]]
+        m = self.fluid[s][idx[0]][:, idx[1]][:, :, idx[2]]
+        if self.rng.random() < 0.5:  # y flip: vy changes sign
+            x, y, m = x.flip(1), y.flip(1), m.flip(1)
+            y = y * torch.tensor([1.0, -1.0, 1.0, 1.0], device=self.device)
+        if self.rng.random() < 0.5:  # z flip: vz changes sign
+            x, y, m = x.flip(2), y.flip(2), m.flip(2)
+            y = y * torch.tensor([1.0, 1.0, -1.0, 1.0], device=self.device)
+        if self.rng.random() < 0.5:  # y-z swap: vy and vz swap
+            x, y, m = x.transpose(1, 2), y.transpose(1, 2), m.transpose(1, 2)
+            y = y[..., [0, 2, 1, 3]]
+        return x.contiguous(), y.contiguous(), m.contiguous()
+
+    def batch(self, n, size):
+        torch = self.torch
+        xs, ys, ms = zip(*(self.crop(size) for _ in range(n)))
+        x = torch.zeros((n, size, size, size, sg.CIN_PAD), dtype=torch.bfloat16, device=self.device)
+        x[..., :2] = torch.stack(xs).to(torch.bfloat16)
+        return x, torch.stack(ys), torch.stack(ms)
+
+
+def train(net, sampler, iters, batch, crop, lr=1e-3, log_every=50):
+    import torch
+    opt = torch.optim.Adam(net.parameters(), lr=lr)
+    losses = []
+    t0 = time.perf_counter()
+    for it in range(iters):
+        x, y, m = sampler.batch(batch, crop)
+        out = net(x)
+        err = (out - y) ** 2 * m[..., None]
+        loss = err.sum() / (m.sum() * 4 + 1e-9)
+        opt.zero_grad(set_to_none=True)
+        loss.backward()
+        opt.step()
+        net.zero_head_padding()
+        losses.append(float(loss.detach()) if (it % log_every == 0 or it == iters - 1) else None)
+        if it % log_every == 0 or it == iters - 1:
+            torch.cuda.synchronize()
+            log(f"iter {it}: loss {losses[-1]:.4g} ({time.perf_counter() - t0:.1f}s)")
+    torch.cuda.synchronize()
+    return [(i, l) for i, l in enumerate(losses) if l is not None], time.perf_counter() - t0
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--train-samples", type=int, default=64)
+    ap.add_argument("--train-seed", type=int, default=1000)
+    ap.add_argument("--iters", type=int, default=1000)
+    ap.add_argument("--batch", type=int, default=2)
+    ap.add_argument("--crop", type=int, default=128)
+    ap.add_argument("--eval-samples", type=int, default=32)
+    ap.add_argument("--eval-size", type=int, default=192)
+    ap.add_argument("--eval-seed", type=int, default=5000)
+    ap.add_argument("--max-steps", type=int, default=50_000)
+    ap.add_argument("--save-model", default="")
+    ap.add_argument("--out", default="")
+    args = ap.parse_args(argv)
+
+    import torch
+    from taichi_lbm3d_amd.geometry import sphere_pack
+    assert torch.cuda.is_available(), "the warm-start pipeline runs on the GPU"
+    dev = torch.device("cuda", torch.cuda.current_device())
+    workers = min(16, len(os.sched_getaffinity(0)))
+    result = {"config": vars(args)}
+
+    # 1. training data from the batched solver
+    t0 = time.time()
+    solids, specs = make_samples(args.train_samples, args.train_seed)
+    b, steps, conv, wall, _ = converge(solids, max_steps=args.max_steps)
+    log(f"training data: {int(conv.sum())}/{len(conv)} converged, steps {steps.min()}-{steps.max()}, {wall:.1f}s")
+    nu = b.effective_viscosity()
+    fields = [b.sample_fields(s) for s in range(b.n_samples)]
+    del b
+    torch.cuda.empty_cache()
+    with ProcessPoolExecutor(workers) as ex:
+        inputs = list(ex.map(_inputs, solids))
+    targets = [sg.make_targets(r, v, s, FX, nu) for (r, v), s in zip(fields, solids)]
+    del fields
+    result["data"] = {"samples": len(solids), "converged": int(conv.sum()), "steps_max": int(steps.max()),
+                      "solver_wall_s": round(wall, 2), "prep_wall_s": round(time.time() - t0, 1)}
+
+    # 2. training
+    net = sg.UNet3D(dev, seed=0)
+    sampler = CropSampler(inputs, targets, solids, dev, seed=1)
+    curve, train_s = train(net, sampler, args.iters, args.batch, args.crop)
+    result["train"] = {"iters": args.iters, "batch": args.batch, "crop": args.crop, "wall_s": round(train_s, 1),
+                       "loss_first": curve[0][1], "loss_last": curve[-1][1], "curve": curve}
+    del sampler, inputs, targets
+    torch.cuda.empty_cache()
+    if args.save_model:
+        torch.save(net.state_dict(), args.save_model)
+
+    # 3. evaluation on held-out packs
+    ev = [sphere_pack((args.eval_size,) * 3, porosity=0.3, rmin=4.0, rmax=12.0, seed=args.eval_seed + i,
+                      periodic=(True, True, True)) for i in range(args.eval_samples)]
+    log(f"evaluation: {len(ev)} packs of {args.eval_size}^3")
+    bc, steps_c, conv_c, wall_c, hist_c = converge(ev, max_steps=args.max_steps)
+    k_cold = bc.permeability()
+    del bc
+    torch.cuda.empty_cache()
+    log(f"cold: {int(conv_c.sum())} converged, steps median {np.median(steps_c):.0f}, {wall_c:.1f}s")
+
+    torch.cuda.synchronize()
+    t_inf = time.perf_counter()
+    preds, fallback = [], np.zeros(len(ev), bool)
+    for i, s in enumerate(ev):
+        rho, v = sg.predict(net, s, FX, nu)
+        if not (np.isfinite(rho).all() and np.isfinite(v).all()):
+            fallback[i] = True
+            rho, v = np.ones(s.shape, np.float32), np.zeros(s.shape + (3,), np.float32)
+        preds.append((rho, v))
+    torch.cuda.synchronize()
+    inf_s = time.perf_counter() - t_inf
+    try:
+        bw, steps_w, conv_w, wall_w, hist_w = converge(ev, max_steps=args.max_steps, init_fields=preds)
+        k_warm = bw.permeability()
+        del bw
+    except FloatingPointError as e:  # the warm run diverged: every sample falls back to the cold start
+        log(f"warm start diverged ({e}); falling back to cold start")
+        fallback[:] = True
+        steps_w, conv_w, wall_w, hist_w, k_warm = steps_c, conv_c, wall_c, hist_c, k_cold
+    fallback |= ~conv_w
+    k_warm = np.where(fallback, k_cold, k_warm)
+    s1_c, w1_c = within(hist_c, k_cold)
+    s1_w, w1_w = within(hist_w, k_cold)
+    err = np.abs(k_warm - k_cold) / np.abs(k_cold)
+    result["eval"] = {
+        "samples": len(ev), "size": args.eval_size, "fallbacks": int(fallback.sum()),
+        "cold": {"steps_median": float(np.median(steps_c)), "steps_max": int(steps_c.max()),
+                 "wall_s": round(wall_c, 2), "converged": int(conv_c.sum()),
+                 "steps_to_1pct_median": float(np.median(s1_c)), "wall_to_1pct_s": float(np.nanmax(w1_c))},
+        "warm": {"steps_median": float(np.median(steps_w)), "steps_max": int(steps_w.max()),
+                 "wall_s": round(wall_w + inf_s, 2), "inference_s": round(inf_s, 2), "converged": int(conv_w.sum()),
+                 "steps_to_1pct_median": float(np.median(s1_w)),
+                 "wall_to_1pct_s": float(np.nanmax(w1_w) + inf_s)},
+        "permeability_rel_err": {"median": float(np.median(err)), "max": float(err.max())},
+        "speedup_to_steady": round(wall_c / (wall_w + inf_s), 3),
+        "speedup_to_1pct": round(float(np.nanmax(w1_c)) / (float(np.nanmax(w1_w)) + inf_s), 3),
+    }
+    log(json.dumps(result["eval"]))
+    line = json.dumps(result)
+    if args.out:
+        with open(args.out, "w") as fh:
+            fh.write(line + "\n")
+    print(line, flush=True)
+    return result
+
+
+if __name__ == "__main__":
+    main()

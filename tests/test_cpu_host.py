@@ -92,9 +92,13 @@ def test_compaction_all_fluid_docs_fixture():
 # ---- the C ABI --------------------------------------------------------------
 
 def _header_functions():
-    txt = open(os.path.join(ROOT, "include", "lbm3d.h")).read()
-    txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
-    return sorted(set(re.findall(r"\b(lbm3d_[a-z_]+)\s*\(", txt)) - {"lbm3d_params"})
+    names = set()
+    for h in sorted(os.listdir(os.path.join(ROOT, "include"))):
+        if h.endswith(".h"):
+            txt = open(os.path.join(ROOT, "include", h)).read()
+            txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+            names |= set(re.findall(r"\b(lbm3d_[a-z0-9_]+)\s*\(", txt))
+    return sorted(names - {"lbm3d_params"})
 
 
 def test_abi_exports_every_declared_symbol():
@@ -164,6 +168,76 @@ def test_abi_rejects_bad_arguments_without_gpu():
     assert lib.cdll.lbm3d_face_bit_words(33) == 3
     assert [lib.cdll.lbm3d_face_plane(ctypes.byref(p), f) for f in range(6)] == [420, 420, 350, 350, 30, 30]
     assert lib.cdll.lbm3d_slot_stride(65) == 128
+
+
+def test_batch_struct_layouts(tmp_path):
+    """ctypes mirrors of lbm3d_sample, lbm3d_batch_sizes and lbm3d_launch have the C layout."""
+    from taichi_lbm3d_amd._lib import BatchSizes, Launch, Sample
+    checks = []
+    for cls, cname in ((Sample, "lbm3d_sample"), (BatchSizes, "lbm3d_batch_sizes"), (Launch, "lbm3d_launch")):
+        checks.append(f'printf("{cname}.size %zu\\n", sizeof({cname}));')
+        for f in cls._fields_:
+            checks.append(f'printf("{cname}.{f[0]} %zu\\n", offsetof({cname}, {f[0]}));')
+    src = tmp_path / "batch_layout.c"
+    src.write_text('#include <stdio.h>\n#include <stddef.h>\n#include "lbm3d.h"\nint main(void){'
+                   + "".join(checks) + 'printf("align %d\\n", LBM3D_BATCH_ALIGN);return 0;}\n')
+    exe = tmp_path / "batch_layout"
+    subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), "-o", str(exe), str(src)], check=True)
+    got = dict(line.split() for line in subprocess.run([str(exe)], capture_output=True, text=True).stdout.split("\n")
+               if line)
+    from taichi_lbm3d_amd import _lib
+    assert int(got["align"]) == _lib.BATCH_ALIGN
+    for cls, cname in ((Sample, "lbm3d_sample"), (BatchSizes, "lbm3d_batch_sizes"), (Launch, "lbm3d_launch")):
+        assert int(got[cname + ".size"]) == ctypes.sizeof(cls)
+        for f in cls._fields_:
+            assert int(got[f"{cname}.{f[0]}"]) == getattr(cls, f[0]).offset, (cname, f[0])
+
+
+def test_batch_layout_host_only():
+    """Samples start on LBM3D_BATCH_ALIGN boundaries in order, each owning at least
+    one chunk; row and face-state blocks are packed in order (no GPU involved)."""
+    from taichi_lbm3d_amd import _lib
+    lib = _lib.lib()
+    p = _lib.Params()
+    p.bc_mode[0] = p.bc_mode[1] = 1          # pressure on both x faces
+    p.bc_mode[4] = 2                         # velocity on z-left: no face state
+    shapes = [(10, 9, 8), (7, 12, 33), (16, 5, 6), (4, 4, 4)]
+    n_cells = [300, 1000, 0, 64]
+    smp = (_lib.Sample * 4)()
+    for i, (sh, n) in enumerate(zip(shapes, n_cells)):
+        smp[i].nx, smp[i].ny, smp[i].nz = sh
+        smp[i].n_cells = n
+    sz = _lib.BatchSizes()
+    lib.call("lbm3d_batch_layout", ctypes.byref(p), smp, 4, ctypes.byref(sz))
+    assert [smp[i].cell_begin for i in range(4)] == [0, 512, 1536, 2048]
+    assert sz.cells == 2560
+    words = [nx * ny * ((nz + 31) // 32) for nx, ny, nz in shapes]
+    assert [smp[i].rows_begin for i in range(4)] == list(np.cumsum([0] + words[:-1]))
+    assert sz.row_words == sum(words)
+    fu = [3 * 2 * ny * nz for _, ny, nz in shapes]
+    assert [smp[i].face_u_begin for i in range(4)] == list(np.cumsum([0] + fu[:-1]))
+    assert sz.face_u == sum(fu)
+    buf = np.zeros(int(sz.desc_bytes), np.uint8)
+    lib.call("lbm3d_batch_desc", ctypes.byref(p), smp, 4, ctypes.c_void_p(buf.ctypes.data), int(sz.desc_bytes))
+    # errors: a sample with a pressure face of extent 1, descriptors of a foreign layout, a short buffer
+    bad = (_lib.Sample * 1)()
+    bad[0].nx, bad[0].ny, bad[0].nz, bad[0].n_cells = 1, 4, 4, 16
+    assert lib.cdll.lbm3d_batch_layout(ctypes.byref(p), bad, 1, ctypes.byref(sz)) == -1
+    assert "extent" in lib.last_error()
+    smp[1].cell_begin += 512
+    assert lib.cdll.lbm3d_batch_desc(ctypes.byref(p), smp, 4, ctypes.c_void_p(buf.ctypes.data), buf.size) == -1
+    smp[1].cell_begin -= 512
+    assert lib.cdll.lbm3d_batch_desc(ctypes.byref(p), smp, 4, ctypes.c_void_p(buf.ctypes.data), 64) == -1
+    # batch calls check the batch cell count before touching the device
+    vp = ctypes.c_void_p(256)
+    p.n_cells = 1000                          # not a multiple of the alignment
+    rc = lib.cdll.lbm3d_batch_step(None, ctypes.byref(p), 0, vp, vp, 4, vp, vp, vp, vp, None, 0, None)
+    assert rc == -1 and "n_cells" in lib.last_error()
+    p.n_cells = 2560
+    shape = (_lib.Launch * 2)()
+    shape[0].block, shape[1].block = 96, 256
+    rc = lib.cdll.lbm3d_batch_step(None, ctypes.byref(p), 0, vp, vp, 4, vp, vp, vp, vp, None, 0, shape)
+    assert rc == -1 and "block" in lib.last_error()
 
 
 # ---- host logic ---------------------------------------------------------------

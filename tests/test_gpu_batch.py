@@ -1,0 +1,182 @@
+"""GPU tests of the batched solver: every sample of a batch evolves bit for
+bit as the single-grid solver evolves it alone (whose parity with the oracle
+the other GPU tests pin), whatever the launch shape, with or without a HIP
+graph, and with retired samples left untouched."""
+import numpy as np
+import pytest
+
+from test_gpu_parity import CASES, case_geometry, make_solver
+
+pytestmark = pytest.mark.gpu
+
+FACE_NAMES = ("x_left", "x_right", "y_left", "y_right", "z_left", "z_right")
+
+
+def batch_case(name, shapes, seed=0, dtype="f32"):
+    """Samples of uneven shapes sharing the physics and face modes of CASES[name]."""
+    from taichi_lbm3d_amd.batch import BatchSolver
+    _, frac, niu, force, modes, rho_bc, u_bc = CASES[name]
+    rng = np.random.default_rng(seed)
+    solids = []
+    for shp in shapes:
+        g = (rng.random(shp) < frac).astype(np.int8)
+        g[(rng.random(shp) < 0.05) & (g == 1)] = -1
+        solids.append(g)
+    faces = {}
+    for f in range(6):
+        if modes[f] == 1:
+            faces[FACE_NAMES[f]] = ("pressure", rho_bc[f])
+        elif modes[f] == 2:
+            faces[FACE_NAMES[f]] = ("velocity", u_bc[f])
+    b = BatchSolver(solids, niu=niu, force=force, faces=faces, dtype=dtype)
+    singles = [make_solver(s, niu, force, modes, rho_bc, u_bc, dtype=dtype) for s in solids]
+    return b, singles
+
+
+def populations(b, s, single):
+    """(batch slice, single-solver array) of sample s's 19 population slots."""
+    import torch
+    from taichi_lbm3d_amd import _lib
+    lib = _lib.lib()
+    n = int(b.samples[s].n_cells)
+    cb = int(b.samples[s].cell_begin)
+    sb = lib.size("lbm3d_slot_stride", int(b.sizes.cells))
+    ss = lib.size("lbm3d_slot_stride", n)
+    gb = b._dev["g"].view(19, sb)[:, cb:cb + n]
+    gs = single._dev["g"].view(19, ss)[:, :n]
+    # raw bytes (numpy has no bfloat16)
+    return gb.contiguous().view(torch.uint8).cpu().numpy(), gs.contiguous().view(torch.uint8).cpu().numpy()
+
+
+def assert_same(b, singles):
+    for s, single in enumerate(singles):
+        gb, gs = populations(b, s, single)
+        np.testing.assert_array_equal(gb, gs, err_msg=f"sample {s}")
+        fu_b = b._dev["face_u"]
+        if int(b.sizes.face_u):
+            begin = int(b.samples[s].face_u_begin)
+            n = single._face_u_floats
+            np.testing.assert_array_equal(fu_b[begin:begin + n].cpu().numpy(),
+                                          single._dev["face_u"][:n].cpu().numpy())
+
+
+SHAPES = [(10, 9, 8), (7, 12, 33), (16, 5, 6), (9, 9, 40)]
+
+
+@pytest.mark.parametrize("name", sorted(CASES))
+@pytest.mark.parametrize("steps", [1, 2, 7, 20])
+def test_batch_bit_exact_vs_single(gpu, name, steps):
+    b, singles = batch_case(name, SHAPES, seed=steps)
+    b.run(steps)
+    for s in singles:
+        s.run(steps)
+    assert_same(b, singles)
+    # macros through the batch path equal the single solver's compact macros
+    rho, v = b.macros()
+    total = int(b.sizes.cells)
+    for s, single in enumerate(singles):
+        cb, n = int(b.samples[s].cell_begin), single.n_fluid
+        r1, v1 = single.compact_macros()
+        np.testing.assert_array_equal(rho[cb:cb + n].cpu().numpy(), r1[:n].cpu().numpy())
+        vb = v.view(3, total)[:, cb:cb + n].cpu().numpy()
+        np.testing.assert_array_equal(vb, v1[: 3 * n].view(3, n).cpu().numpy())
+
+
+@pytest.mark.parametrize("shape", [(128, 0, 0), (256, 0, 1), (512, 0, 0), (128, 2, 1), (256, 4, 0), (512, 8, 1)])
+def test_batch_launch_shapes_bit_exact(gpu, shape):
+    """Every launch shape (block, persistent grid, sweep order) gives the same bits."""
+    b, singles = batch_case("mixed_all_faces", SHAPES + [(20, 20, 20)], seed=3)
+    b.set_launch(shape, (shape[0], shape[1], 1 - shape[2]))
+    b.run(9)
+    for s in singles:
+        s.run(9)
+    assert_same(b, singles)
+
+
+def test_batch_graph_replay(gpu):
+    """Steps replayed from a captured HIP graph equal eager steps, replay after
+    replay, with eager work and synchronisations in between."""
+    import torch
+    a, _ = batch_case("periodic_force", SHAPES, seed=5)
+    g, _ = batch_case("periodic_force", SHAPES, seed=5)
+    for rnd in range(4):
+        a.run(13)
+        g.run(13, graph_steps=4)
+        assert a.steps_done == g.steps_done
+        torch.cuda.synchronize()
+        np.testing.assert_array_equal(a._dev["g"].view(torch.uint8).cpu().numpy(),
+                                      g._dev["g"].view(torch.uint8).cpu().numpy(), err_msg=f"round {rnd}")
+        ra, rg = a.reduce(), g.reduce()  # eager kernels between replays
+        np.testing.assert_array_equal(ra, rg)
+
+
+def test_batch_retire_freezes_sample(gpu):
+    b, singles = batch_case("pressure_force", SHAPES, seed=9)
+    b.run(6)
+    for s in singles:
+        s.run(6)
+    frozen = populations(b, 1, singles[1])[0].copy()
+    b.retire([False, True, False, False])
+    b.run(10)
+    for i, s in enumerate(singles):
+        if i != 1:
+            s.run(10)
+    np.testing.assert_array_equal(populations(b, 1, singles[1])[0], frozen)
+    for i in (0, 2, 3):
+        gb, gs = populations(b, i, singles[i])
+        np.testing.assert_array_equal(gb, gs)
+
+
+def test_batch_reduce_matches_single(gpu):
+    b, singles = batch_case("periodic_force", SHAPES, seed=2)
+    b.run(15)
+    r = b.reduce()
+    for s, single in enumerate(singles):
+        single.run(15)
+        rs = single.reduce()
+        assert int(r[s, 6]) == single.n_fluid
+        assert r[s, 5] == 0
+        np.testing.assert_allclose(r[s, 0], rs["sum_rho"], rtol=1e-13)
+        np.testing.assert_allclose(r[s, 1:4], rs["sum_v"], rtol=1e-11, atol=1e-15)
+        np.testing.assert_allclose(r[s, 4], rs["max_v"], rtol=1e-15)
+
+
+@pytest.mark.parametrize("dtype", ["f16", "bf16", "f64"])
+def test_batch_storage_types_bit_exact(gpu, dtype):
+    b, singles = batch_case("pressure_x", SHAPES, seed=4, dtype=dtype)
+    b.run(8)
+    for s in singles:
+        s.run(8)
+    assert_same(b, singles)
+
+
+def test_batch_sample_fields_and_warm_start(gpu):
+    b, singles = batch_case("periodic_force", SHAPES, seed=7)
+    rng = np.random.default_rng(0)
+    fields = []
+    for s in b.solids:
+        fields.append(((1 + 1e-3 * rng.standard_normal(s.shape)).astype(np.float32),
+                       (1e-3 * rng.standard_normal(s.shape + (3,))).astype(np.float32)))
+    b.init_from_macros(fields)
+    for single, (r, v) in zip(singles, fields):
+        single.init_from_macros(r, v)
+    b.run(5)
+    for i, single in enumerate(singles):
+        single.run(5)
+        rho, v = b.sample_fields(i)
+        np.testing.assert_array_equal(rho, single.rho.to_numpy())
+        np.testing.assert_array_equal(v, single.v.to_numpy())
+
+
+def test_autotune_picks_valid_shapes(gpu, tmp_path, monkeypatch):
+    from taichi_lbm3d_amd import autotune
+    monkeypatch.setenv("TAICHI_LBM3D_TUNE_CACHE", "0")
+    b, singles = batch_case("periodic_force", [(24, 24, 24), (30, 20, 16)], seed=1)
+    res = autotune.tune(b, candidates=[(128, 0, 0), (256, 0, 1), (512, 2, 0)], pairs=2)
+    assert res["even"] in [(128, 0, 0), (256, 0, 1), (512, 2, 0)]
+    assert len(res["table"]) == 3
+    t = b.steps_done
+    b.run(4)
+    for s in singles:
+        s.run(t + 4)
+    assert_same(b, singles)
