@@ -12,7 +12,7 @@ import subprocess
 _HERE = os.path.dirname(os.path.abspath(__file__))
 _ROOT = os.path.dirname(_HERE)
 LIB_PATH = os.path.join(_HERE, "liblbm3d.so")
-SOURCES = [os.path.join(_HERE, "csrc", "lbm3d.hip"), os.path.join(_HERE, "csrc", "conv3d.hip")]
+SOURCES = [os.path.join(_HERE, "csrc", name) for name in ("lbm3d.hip", "conv3d.hip", "geo_io.hip")]
 INCLUDE = os.path.join(_ROOT, "include")
 
 ABI_VERSION = 6
@@ -123,6 +123,11 @@ SIGNATURES = {
     "lbm3d_conv3d_dgrad": (_int, [_vp, _vp, _vp, _vp, _i32, _i32, _i32, _i32, _i32, _i32, _i32]),
     "lbm3d_conv3d_wgrad_workspace_bytes": (_sz, [_i32, _i32, _i32, _i32, _i32, _i32, _i32]),
     "lbm3d_conv3d_wgrad": (_int, [_vp, _vp, _vp, _vp, _i32, _i32, _i32, _i32, _i32, _i32, _i32, _vp, _sz]),
+    # include/lbm3d_io.h
+    "lbm3d_io_last_error": (ctypes.c_char_p, []),
+    "lbm3d_read_geo_ascii": (_i64, [ctypes.c_char_p, _i32, _i32, _i32, _vp]),
+    "lbm3d_voxelize": (_int, [_vp, _vp, _i32, ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                              _i32, _i32, _i32, _vp]),
 }
 
 

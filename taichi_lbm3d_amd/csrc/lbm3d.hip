@@ -34,6 +34,7 @@ constexpr int kQ = 19;
 #define LBM3D_BLOCK 256
 #endif
 constexpr int kBlock = LBM3D_BLOCK;  // threads per block, a multiple of 64
+constexpr int kWinWords = 384;       // LDS row window: 3 planes x (J+3) rows x W words
 
 // Build-time knobs (defaults are the measured best on MI355X).
 #ifndef LBM3D_XCD_REMAP_EVEN
@@ -50,6 +51,9 @@ constexpr int kBlock = LBM3D_BLOCK;  // threads per block, a multiple of 64
 // preceding odd step wrote last, while those are still in the 256 MB last-level
 // cache; the next odd step, walking forwards, likewise starts where this one ended.
 #define LBM3D_REVERSE_EVEN 1
+#endif
+#ifndef LBM3D_ROW_WINDOW
+#define LBM3D_ROW_WINDOW 1    // odd steps read row words from a per-workgroup LDS window
 #endif
 #ifndef LBM3D_CPOL_LOAD
 #define LBM3D_CPOL_LOAD 0     // cache-policy bits of population loads
@@ -250,10 +254,8 @@ __device__ __forceinline__ int32_t id_in_word(uint4 w, int k) {
   return fl ? (int32_t)(w.y + __popc(w.x & ((1u << b) - 1u))) : -1;
 }
 
-// ids of cells (k-1, k, k+1) of one row, k-1 and k+1 wrapped, from one word
-__device__ __forceinline__ void row3(const Rows &rb, const Geo &gm, uint32_t row, int k, int32_t &lo,
-                                     int32_t &mid, int32_t &hi) {
-  const uint4 w = rb.word(row, gm.W, k);
+// ids of cells (k-1, k, k+1) of one row, k-1 and k+1 wrapped, from its word w
+__device__ __forceinline__ void row3(const uint4 w, const Geo &gm, int k, int32_t &lo, int32_t &mid, int32_t &hi) {
   const uint32_t b = (uint32_t)k & 31u;
   const uint32_t below = w.y + __popc(w.x & ((1u << b) - 1u));
   const uint32_t fm = (w.x >> b) & 1u;
@@ -265,44 +267,72 @@ __device__ __forceinline__ void row3(const Rows &rb, const Geo &gm, uint32_t row
   lo = b == 0u ? (int32_t)w.z : (down_fluid ? (int32_t)below - 1 : -1);
 }
 
+// The 9 row words around a cell (i, j): word(di, dj, k) is the word holding k
+// of row (i + di, j + dj), wrapped (periodic_index, SP:247-257).
+// From the row table in memory:
+struct TableRows {
+  const Rows &rb;
+  uint32_t W, ny, ri[3], rj[3];
+  __device__ __forceinline__ TableRows(const Rows &r, const Geo &gm, int i, int j) : rb(r) {
+    W = gm.W;
+    ny = (uint32_t)gm.ny;
+    ri[0] = (uint32_t)wrap(i - 1, gm.nx);
+    ri[1] = (uint32_t)i;
+    ri[2] = (uint32_t)wrap(i + 1, gm.nx);
+    rj[0] = (uint32_t)wrap(j - 1, gm.ny);
+    rj[1] = (uint32_t)j;
+    rj[2] = (uint32_t)wrap(j + 1, gm.ny);
+  }
+  __device__ __forceinline__ uint4 word(int di, int dj, int k) const {
+    return rb.word(ri[di + 1] * ny + rj[dj + 1], W, k);
+  }
+};
+// From a workgroup's LDS copy of the rows i0-1 .. i0+1 x j0-1 .. j0+J+1 (JW = J + 3
+// rows per plane), for a cell of row (i0, j0 + jrel):
+struct WindowRows {
+  const uint4 *win;
+  uint32_t W, JW;
+  int jrel;
+  __device__ __forceinline__ uint4 word(int di, int dj, int k) const {
+    return win[((uint32_t)(di + 1) * JW + (uint32_t)(jrel + dj + 1)) * W + ((uint32_t)k >> 5)];
+  }
+};
+
 // Compact ids of the 18 neighbours (index q-1), -1 where the neighbour is
 // not fluid: the push target of streaming1 (SP:264-266) or its bounce-back
 // (SP:267-268).
-__device__ __forceinline__ void neighbor_ids(const Rows &rb, const Geo &gm, int i, int j, int k, int32_t nb[18]) {
-  const uint32_t ny = (uint32_t)gm.ny;
-  const uint32_t ip = (uint32_t)wrap(i + 1, gm.nx), im = (uint32_t)wrap(i - 1, gm.nx);
-  const uint32_t jp = (uint32_t)wrap(j + 1, gm.ny), jm = (uint32_t)wrap(j - 1, gm.ny);
-  const uint32_t ii = (uint32_t)i, jj = (uint32_t)j;
+template <class Src>
+__device__ __forceinline__ void neighbor_ids(const Src &src, const Geo &gm, int k, int32_t nb[18]) {
   int32_t a, b, c;
   // own row: q5 (0,0,1), q6 (0,0,-1)
-  row3(rb, gm, ii * ny + jj, k, a, b, c);
+  row3(src.word(0, 0, k), gm, k, a, b, c);
   nb[4] = c;
   nb[5] = a;
   // row (i+1, j): q1 (1,0,0), q11 (1,0,1), q13 (1,0,-1)
-  row3(rb, gm, ip * ny + jj, k, a, b, c);
+  row3(src.word(1, 0, k), gm, k, a, b, c);
   nb[0] = b;
   nb[10] = c;
   nb[12] = a;
   // row (i-1, j): q2 (-1,0,0), q12 (-1,0,-1), q14 (-1,0,1)
-  row3(rb, gm, im * ny + jj, k, a, b, c);
+  row3(src.word(-1, 0, k), gm, k, a, b, c);
   nb[1] = b;
   nb[11] = a;
   nb[13] = c;
   // row (i, j+1): q3 (0,1,0), q15 (0,1,1), q17 (0,1,-1)
-  row3(rb, gm, ii * ny + jp, k, a, b, c);
+  row3(src.word(0, 1, k), gm, k, a, b, c);
   nb[2] = b;
   nb[14] = c;
   nb[16] = a;
   // row (i, j-1): q4 (0,-1,0), q16 (0,-1,-1), q18 (0,-1,1)
-  row3(rb, gm, ii * ny + jm, k, a, b, c);
+  row3(src.word(0, -1, k), gm, k, a, b, c);
   nb[3] = b;
   nb[15] = a;
   nb[17] = c;
   // diagonal rows: q7 (1,1,0), q8 (-1,-1,0), q9 (1,-1,0), q10 (-1,1,0)
-  nb[6] = id_in_word(rb.word(ip * ny + jp, gm.W, k), k);
-  nb[7] = id_in_word(rb.word(im * ny + jm, gm.W, k), k);
-  nb[8] = id_in_word(rb.word(ip * ny + jm, gm.W, k), k);
-  nb[9] = id_in_word(rb.word(im * ny + jp, gm.W, k), k);
+  nb[6] = id_in_word(src.word(1, 1, k), k);
+  nb[7] = id_in_word(src.word(-1, -1, k), k);
+  nb[8] = id_in_word(src.word(1, -1, k), k);
+  nb[9] = id_in_word(src.word(-1, 1, k), k);
 }
 
 // ---------------------------------------------------------------------------
@@ -584,9 +614,18 @@ struct CellCtx {
   bool have_coords;
 };
 
+// Row words for the odd-step neighbour lookup: the table in memory, or the
+// workgroup's LDS window when win != nullptr.
+struct RowView {
+  const Rows *rb;
+  const uint4 *win;
+  uint32_t JW;
+  int j0;
+};
+
 template <typename T, bool ODD>
 __device__ __forceinline__ void read_cell(const Slots<T> &sl, const Geo &gm, const int32_t *__restrict__ cells,
-                                          const Rows &rb, uint32_t x, typename St<T>::R v[kQ], uint32_t off[kQ],
+                                          const RowView &rv, uint32_t x, typename St<T>::R v[kQ], uint32_t off[kQ],
                                           CellCtx &cc) {
   const uint32_t xb = x * (uint32_t)sizeof(T);
   if constexpr (!ODD) {
@@ -597,7 +636,13 @@ __device__ __forceinline__ void read_cell(const Slots<T> &sl, const Geo &gm, con
     coords(gm, (uint32_t)cells[x], cc.i, cc.j, cc.k);
     cc.have_coords = true;
     int32_t nb[18];
-    neighbor_ids(rb, gm, cc.i, cc.j, cc.k, nb);
+    if (rv.win) {
+      const WindowRows src{rv.win, gm.W, rv.JW, cc.j - rv.j0};
+      neighbor_ids(src, gm, cc.k, nb);
+    } else {
+      const TableRows src(*rv.rb, gm, cc.i, cc.j);
+      neighbor_ids(src, gm, cc.k, nb);
+    }
     off[0] = xb;
 #pragma unroll
     for (int q = 1; q < kQ; ++q) {
@@ -620,7 +665,7 @@ __device__ __forceinline__ uint32_t face_record(const uint32_t *__restrict__ fac
 // One fluid cell's time step: read (with the previous step's face overwrite),
 // macros, pressure-face state, collision, write back in place.
 template <typename T, bool ODD, bool FORCED>
-__device__ __forceinline__ void step_cell(const KP &kp, const Geo &gm, const Slots<T> &sl, const Rows &rb,
+__device__ __forceinline__ void step_cell(const KP &kp, const Geo &gm, const Slots<T> &sl, const RowView &rv,
                                           const int32_t *__restrict__ cells, const uint32_t *__restrict__ face_bits,
                                           typename St<T>::R *__restrict__ face_u, uint32_t x, int first) {
   using R = typename St<T>::R;
@@ -629,7 +674,7 @@ __device__ __forceinline__ void step_cell(const KP &kp, const Geo &gm, const Slo
   R v[kQ];
   uint32_t off[kQ];
   CellCtx cc;
-  read_cell<T, ODD>(sl, gm, cells, rb, x, v, off, cc);
+  read_cell<T, ODD>(sl, gm, cells, rv, x, v, off, cc);
 
   const uint32_t fr = face_record(face_bits, x);
   int owner = -1;
@@ -666,7 +711,7 @@ __device__ __forceinline__ void step_cell(const KP &kp, const Geo &gm, const Slo
 
 // rho and v of one fluid cell as streaming3 leaves them after the last step
 template <typename T, bool ODD>
-__device__ __forceinline__ void macros_cell(const KP &kp, const Geo &gm, const Slots<T> &sl, const Rows &rb,
+__device__ __forceinline__ void macros_cell(const KP &kp, const Geo &gm, const Slots<T> &sl, const RowView &rv,
                                             const int32_t *__restrict__ cells, const uint32_t *__restrict__ face_bits,
                                             const typename St<T>::R *__restrict__ face_u, uint32_t x, int first,
                                             typename St<T>::R *__restrict__ rho_out,
@@ -675,7 +720,7 @@ __device__ __forceinline__ void macros_cell(const KP &kp, const Geo &gm, const S
   R v[kQ];
   uint32_t off[kQ];
   CellCtx cc;
-  read_cell<T, ODD>(sl, gm, cells, rb, x, v, off, cc);
+  read_cell<T, ODD>(sl, gm, cells, rv, x, v, off, cc);
   const uint32_t fr = face_record(face_bits, x);
   if ((fr & 1u) && !first) {
     if (!cc.have_coords) coords(gm, (uint32_t)cells[x], cc.i, cc.j, cc.k);
@@ -726,12 +771,40 @@ template <typename T, bool ODD, bool FORCED>
 __global__ void __launch_bounds__(kBlock, LBM3D_STEP_MINB)
     k_step(const KP kp, T *__restrict__ g, const int32_t *__restrict__ cells, const uint4 *__restrict__ rows,
            const uint32_t *__restrict__ face_bits, typename St<T>::R *__restrict__ face_u, int first) {
-  const uint32_t x = logical_block<ODD ? LBM3D_XCD_REMAP_ODD : LBM3D_XCD_REMAP_EVEN,
-                                   !ODD && LBM3D_REVERSE_EVEN>() * kBlock + threadIdx.x;
-  if (x >= (uint32_t)kp.p.n_cells) return;
+  const uint32_t x0 = logical_block<ODD ? LBM3D_XCD_REMAP_ODD : LBM3D_XCD_REMAP_EVEN,
+                                    !ODD && LBM3D_REVERSE_EVEN>() * kBlock;
+  const uint32_t n = (uint32_t)kp.p.n_cells;
+  if (x0 >= n) return;  // whole padding blocks of the grid
+  const uint32_t x = x0 + threadIdx.x;
   const Slots<T> sl(g, kp.S);
   const Rows rb(rows, kp.geo.n_row_words);
-  step_cell<T, ODD, FORCED>(kp, kp.geo, sl, rb, cells, face_bits, face_u, x, first);
+  RowView rv{&rb, nullptr, 0u, 0};
+  if constexpr (ODD && LBM3D_ROW_WINDOW) {
+    // A workgroup's cells lie in a few consecutive rows (i0, j0 .. j0+J) of one
+    // plane: stage the rows i0-1 .. i0+1 x j0-1 .. j0+J+1 of the table in LDS so
+    // each cell's 9 row words are LDS reads, not global loads.
+    __shared__ uint4 win[kWinWords];
+    const Geo &gm = kp.geo;
+    const uint32_t xl = x0 + kBlock - 1 < n ? x0 + kBlock - 1 : n - 1;
+    int i0, j0, k0, i1, j1, k1;
+    coords(gm, (uint32_t)cells[x0], i0, j0, k0);
+    coords(gm, (uint32_t)cells[xl], i1, j1, k1);
+    const uint32_t JW = (uint32_t)(j1 - j0) + 3u;
+    if (i0 == i1 && JW * 3u * gm.W <= (uint32_t)kWinWords) {
+      const uint32_t per_plane = JW * gm.W, cnt = 3u * per_plane;
+      for (uint32_t e = threadIdx.x; e < cnt; e += kBlock) {
+        const uint32_t di = e / per_plane, r = e - di * per_plane;
+        const uint32_t dj = r / gm.W, w = r - dj * gm.W;
+        const uint32_t row = (uint32_t)wrap(i0 + (int)di - 1, gm.nx) * (uint32_t)gm.ny +
+                             (uint32_t)wrap(j0 + (int)dj - 1, gm.ny);
+        win[e] = rows[row * gm.W + w];
+      }
+      __syncthreads();
+      rv = RowView{&rb, win, JW, j0};
+    }
+  }
+  if (x >= n) return;
+  step_cell<T, ODD, FORCED>(kp, kp.geo, sl, rv, cells, face_bits, face_u, x, first);
 }
 
 template <typename T, bool ODD>
@@ -745,7 +818,8 @@ __global__ void __launch_bounds__(kBlock)
   if (x >= N) return;
   const Slots<T> sl(g, kp.S);
   const Rows rb(rows, kp.geo.n_row_words);
-  macros_cell<T, ODD>(kp, kp.geo, sl, rb, cells, face_bits, face_u, x, first, rho_out, v_out, N);
+  const RowView rv{&rb, nullptr, 0u, 0};
+  macros_cell<T, ODD>(kp, kp.geo, sl, rv, cells, face_bits, face_u, x, first, rho_out, v_out, N);
 }
 
 template <typename T>
@@ -799,7 +873,8 @@ __global__ void __launch_bounds__(B)
     const uint32_t x = x0 + threadIdx.x;
     if (x >= d.cell_end) continue;
     const Rows rb(rows + d.rows_begin, d.geo.n_row_words);
-    step_cell<T, ODD, FORCED>(kp, d.geo, sl, rb, cells, face_bits, face_u, x, first);
+    const RowView rv{&rb, nullptr, 0u, 0};
+    step_cell<T, ODD, FORCED>(kp, d.geo, sl, rv, cells, face_bits, face_u, x, first);
   }
 }
 
@@ -815,7 +890,8 @@ __global__ void __launch_bounds__(kBlock)
   if (x >= d.cell_end) return;
   const Slots<T> sl(g, kp.S);
   const Rows rb(rows + d.rows_begin, d.geo.n_row_words);
-  macros_cell<T, ODD>(kp, d.geo, sl, rb, cells, face_bits, face_u, x, first, rho_out, v_out, (uint32_t)kp.p.n_cells);
+  const RowView rv{&rb, nullptr, 0u, 0};
+  macros_cell<T, ODD>(kp, d.geo, sl, rv, cells, face_bits, face_u, x, first, rho_out, v_out, (uint32_t)kp.p.n_cells);
 }
 
 template <typename T>
@@ -984,7 +1060,7 @@ __global__ void __launch_bounds__(kBlock) k_neighbors(const KP kp, const int32_t
   coords(kp.geo, (uint32_t)cells[x], i, j, k);
   int32_t nb[18];
   const Rows rb(rows, kp.geo.n_row_words);
-  neighbor_ids(rb, kp.geo, i, j, k, nb);
+  neighbor_ids(TableRows(rb, kp.geo, i, j), kp.geo, k, nb);
 #pragma unroll
   for (int q = 0; q < 18; ++q) nbr[(size_t)q * N + x] = nb[q];
 }

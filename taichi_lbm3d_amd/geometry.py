@@ -11,17 +11,21 @@ import numpy as np
 def read_geo(path: str, nx: int, ny: int, nz: int) -> np.ndarray:
     """ASCII voxel file -> solid mask, as init_geo does (SP:173-177).
 
-    One integer per line, x fastest, then y, then z (README.md:17-23); any
-    value > 0 becomes 1.  Loaded with ``np.loadtxt`` then reshaped in Fortran
-    order to ``(nx, ny, nz)`` and cast to int8 like the i8 ``solid`` field, so
-    a negative value stays negative: such a cell is not fluid (the kernels test
+    One number per line, x fastest, then y, then z (README.md:17-23); any value
+    > 0 becomes 1 and the field is int8 like the reference's i8 ``solid``
+    (np.loadtxt, a Fortran-order reshape to (nx, ny, nz) and a cast), so a
+    negative value stays negative: such a cell is not fluid (the kernels test
     ``solid == 0``) yet not solid for the pressure-face test ``solid > 0``.
+    Parsed by the library's native reader (lbm3d_read_geo_ascii).
     """
-    data = np.loadtxt(path)
-    data[data > 0] = 1
-    if data.size != nx * ny * nz:
-        raise ValueError(f"{path}: {data.size} values, expected {nx}*{ny}*{nz}")
-    return np.ascontiguousarray(np.reshape(data, (nx, ny, nz), order="F")).astype(np.int8)
+    import ctypes
+    from . import _lib
+    out = np.empty((nx, ny, nz), np.int8)
+    lib = _lib.lib()
+    n = lib.cdll.lbm3d_read_geo_ascii(str(path).encode(), nx, ny, nz, out.ctypes.data_as(ctypes.c_void_p))
+    if n < 0:
+        raise ValueError(lib.cdll.lbm3d_io_last_error().decode())
+    return out
 
 
 def write_geo(path: str, solid: np.ndarray) -> None:

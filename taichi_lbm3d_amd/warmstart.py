@@ -60,6 +60,8 @@ def converge(solids, graph_steps=100, max_steps=50_000, tol=1e-6, init_fields=No
 
     def cb(t, r):
         hist.append((t, time.perf_counter() - t0, b.permeability(r)))
+        if len(hist) % 10 == 0:
+            log(f"  step {t}: {time.perf_counter() - t0:.1f}s")
 
     steps, conv = b.run_to_steady(tol=tol, interval=500, max_steps=max_steps, graph_steps=graph_steps, callback=cb)
     torch.cuda.synchronize()
@@ -155,24 +157,27 @@ def main(argv=None):
     ap.add_argument("--out", default="")
     args = ap.parse_args(argv)
 
+    result = {"config": vars(args)}
+    # worker processes first (geometry, distance fields): nothing may fork after the GPU is initialised
+    t0 = time.time()
+    workers = min(16, len(os.sched_getaffinity(0)))
+    solids, specs = make_samples(args.train_samples, args.train_seed, workers)
+    with ProcessPoolExecutor(workers) as ex:
+        inputs = list(ex.map(_inputs, solids))
+    log(f"{len(solids)} training geometries and distance fields ({time.time() - t0:.1f}s)")
+
     import torch
     from taichi_lbm3d_amd.geometry import sphere_pack
     assert torch.cuda.is_available(), "the warm-start pipeline runs on the GPU"
     dev = torch.device("cuda", torch.cuda.current_device())
-    workers = min(16, len(os.sched_getaffinity(0)))
-    result = {"config": vars(args)}
 
     # 1. training data from the batched solver
-    t0 = time.time()
-    solids, specs = make_samples(args.train_samples, args.train_seed)
     b, steps, conv, wall, _ = converge(solids, max_steps=args.max_steps)
     log(f"training data: {int(conv.sum())}/{len(conv)} converged, steps {steps.min()}-{steps.max()}, {wall:.1f}s")
     nu = b.effective_viscosity()
     fields = [b.sample_fields(s) for s in range(b.n_samples)]
     del b
     torch.cuda.empty_cache()
-    with ProcessPoolExecutor(workers) as ex:
-        inputs = list(ex.map(_inputs, solids))
     targets = [sg.make_targets(r, v, s, FX, nu) for (r, v), s in zip(fields, solids)]
     del fields
     result["data"] = {"samples": len(solids), "converged": int(conv.sum()), "steps_max": int(steps.max()),

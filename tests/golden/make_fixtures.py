@@ -11,6 +11,12 @@ SHA-256 of the fluid list and neighbour table, plus the face records, that
 the plain-loop restatement (oracle.compaction.compact_loops) produces for the
 cavity mask with the example_cavity.py boundary (velocity on x-right,
 SP:405-407).
+
+pack32_compaction.npz holds a seeded 32^3 overlapping-sphere pack (this
+package's generator, seed 7, radii 3-6, porosity 0.35, periodic in y and z)
+bit-packed in C order, with the fluid list and the full neighbour table from
+compact_loops for pressure faces on x: the checksums for a porous medium, where
+bounce-back and periodic wrap both occur.
 """
 import hashlib
 import os
@@ -45,6 +51,14 @@ def main():
                         sha_cells=hashlib.sha256(cells.astype("<i4").tobytes()).hexdigest(),
                         sha_nbr=hashlib.sha256(nbr.astype("<i4").tobytes()).hexdigest())
     print("wrote fixtures:", cells.size, "fluid cells,", int((rec > 0).sum()), "face-owned")
+    from taichi_lbm3d_amd.geometry import sphere_pack
+    pack = sphere_pack((32, 32, 32), porosity=0.35, rmin=3.0, rmax=6.0, seed=7, periodic=(False, True, True))
+    modes = (1, 1, 0, 0, 0, 0)
+    cells, rec, nbr = compact_loops(pack, modes)
+    np.savez_compressed(os.path.join(HERE, "pack32_compaction.npz"), bits=np.packbits(pack.reshape(-1)),
+                        modes=np.array(modes), cells=cells.astype(np.int32), nbr=nbr.astype(np.int32),
+                        face_rec=rec.astype(np.uint8))
+    print("pack32:", cells.size, "fluid cells")
 
 
 if __name__ == "__main__":

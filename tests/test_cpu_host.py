@@ -89,6 +89,21 @@ def test_compaction_all_fluid_docs_fixture():
         assert np.unique(nbr[q]).size == cells.size
 
 
+def test_compaction_pack32_fixture():
+    """Seeded 32^3 sphere pack: the generator is deterministic and the vectorised
+    compaction and neighbour table equal the plain-loop ones stored in the fixture."""
+    from taichi_lbm3d_amd.geometry import sphere_pack
+    g = np.load(os.path.join(GOLDEN, "pack32_compaction.npz"))
+    solid = np.unpackbits(g["bits"])[: 32 ** 3].reshape(32, 32, 32)
+    np.testing.assert_array_equal(solid, sphere_pack((32, 32, 32), porosity=0.35, rmin=3.0, rmax=6.0, seed=7,
+                                                     periodic=(False, True, True)))
+    cells, d2c = oc.compact(solid)
+    np.testing.assert_array_equal(cells, g["cells"])
+    np.testing.assert_array_equal(oc.neighbors(solid, cells, d2c), g["nbr"])
+    np.testing.assert_array_equal(oc.face_records(solid, cells, tuple(g["modes"])), g["face_rec"])
+    assert (g["nbr"] < 0).any() and (g["nbr"] >= 0).any()
+
+
 # ---- the C ABI --------------------------------------------------------------
 
 def _header_functions():
@@ -322,3 +337,56 @@ def test_vtk_roundtrip(tmp_path):
     np.testing.assert_array_equal(back["x"], x)
     head = open(fn, "rb").read(400).decode("ascii", "replace")
     assert 'type="RectilinearGrid"' in head and 'WholeExtent="0 4 0 3 0 2"' in head
+
+
+# ---- geometry input -----------------------------------------------------------------
+
+def test_native_geo_reader_matches_loadtxt(tmp_path):
+    """lbm3d_read_geo_ascii == the reference's np.loadtxt path (oracle) on the
+    one-value-per-line format: integers, fractions, exponents, negatives, CRLF."""
+    from oracle.geo_ref import read_geo_loadtxt
+    from taichi_lbm3d_amd.geometry import read_geo
+    rng = np.random.default_rng(3)
+    shape = (7, 5, 6)
+    vals = rng.choice(["0", "1", "2", "-1", "0.5", "-0.5", "-1.7", "1e-3", "0.0", "+1", "3.0E0"], size=np.prod(shape))
+    p = tmp_path / "g.dat"
+    p.write_text("\n".join(vals) + "\n")
+    np.testing.assert_array_equal(read_geo(str(p), *shape), read_geo_loadtxt(str(p), *shape))
+    p.write_text("\r\n".join(" " + v + "\t" for v in vals) + "\r\n\n")  # CRLF, padding, blank last line
+    np.testing.assert_array_equal(read_geo(str(p), *shape), read_geo_loadtxt(str(p), *shape))
+    with pytest.raises(ValueError):
+        read_geo(str(p), 7, 5, 5)
+    p.write_text("0\n1\nx\n")
+    with pytest.raises(ValueError):
+        read_geo(str(p), 3, 1, 1)
+
+
+def test_stl_read_write_roundtrip(tmp_path):
+    from taichi_lbm3d_amd.stl import read_stl, sphere_mesh, write_stl
+    tris = sphere_mesh((1.0, 2.0, 3.0), 2.5, subdivisions=2)
+    write_stl(str(tmp_path / "s.stl"), tris)
+    np.testing.assert_array_equal(read_stl(str(tmp_path / "s.stl")), tris)
+    lines = ["solid s"]
+    for t in tris:
+        lines += ["facet normal 0 0 0", " outer loop"] + [f"  vertex {x!r} {y!r} {z!r}" for x, y, z in t.tolist()]
+        lines += [" endloop", "endfacet"]
+    (tmp_path / "a.stl").write_text("\n".join(lines + ["endsolid s"]))
+    np.testing.assert_allclose(read_stl(str(tmp_path / "a.stl")), tris, rtol=1e-6)
+
+
+def test_winding_number_oracle_closed_meshes():
+    """w = 1 inside and 0 outside a closed outward mesh (box, icosphere); this pins
+    the solid-angle formula and the meshes' orientation used by the GPU tests."""
+    from oracle.geo_ref import winding_number
+    from taichi_lbm3d_amd.stl import box_mesh, sphere_mesh
+    rng = np.random.default_rng(0)
+    box = box_mesh((0.0, 0.0, 0.0), (2.0, 3.0, 1.5))
+    pts = rng.uniform(-1, 4, (400, 3))
+    inside = np.all((pts > 0) & (pts < np.array([2.0, 3.0, 1.5])), axis=1)
+    w = winding_number(box, pts)
+    np.testing.assert_allclose(w, inside.astype(float), atol=1e-9)
+    sph = sphere_mesh((0.5, -0.5, 1.0), 2.0, subdivisions=3)
+    r = np.linalg.norm(pts - np.array([0.5, -0.5, 1.0]), axis=1)
+    w = winding_number(sph, pts)
+    far = np.abs(r - 2.0) > 0.05
+    np.testing.assert_allclose(w[far], (r[far] < 2.0).astype(float), atol=1e-9)

@@ -112,6 +112,19 @@ def test_compaction_bit_exact(gpu, name):
     np.testing.assert_array_equal(cells_g.cpu().numpy(), cells)
 
 
+def test_compaction_pack32_fixture(gpu):
+    """The GPU fluid list, neighbour ids and face records of the seeded 32^3 pack
+    equal the plain-loop restatement's, stored in tests/golden."""
+    g = np.load(os.path.join(os.path.dirname(__file__), "golden", "pack32_compaction.npz"))
+    solid = np.unpackbits(g["bits"])[: 32 ** 3].reshape(32, 32, 32).astype(np.int8)
+    modes = tuple(int(m) for m in g["modes"])
+    s = make_solver(solid, 0.1, (0, 0, 0), modes, (1.001, 1.0, 1, 1, 1, 1), [(0, 0, 0)] * 6)
+    np.testing.assert_array_equal(s._dev["cells"][: s.n_fluid].cpu().numpy(), g["cells"])
+    np.testing.assert_array_equal(s.neighbor_table(), g["nbr"])
+    words = oc.pack_face_bits(g["face_rec"])
+    np.testing.assert_array_equal(s._dev["face_bits"][: words.size].cpu().numpy().view(np.uint32), words)
+
+
 def test_compaction_cavity_fixture(gpu):
     """The reference's 50^3 cavity mask: 112,896 fluid cells (Single_phase/geo_cavity.dat)."""
     import hashlib
