@@ -17,8 +17,9 @@
  *   bias         FP32 b[cout] (may be NULL);
  *   weight grads FP32 dw[co][kpad].
  * Convolutions are "same" (stride 1, zero padding ks/2).  cin and cout must
- * be multiples of 16, at most 256.  Return 0 or a negative LBM3D_E* code
- * (lbm3d.h); lbm3d_last_error() has no entry for these, lbm3d_conv_last_error() does.
+ * be multiples of 16, at most 256; a batch holds fewer than 2^31 voxels.
+ * Return 0 or a negative LBM3D_E* code (lbm3d.h); lbm3d_conv_last_error()
+ * gives the message.
  */
 #ifndef LBM3D_CONV_H
 #define LBM3D_CONV_H

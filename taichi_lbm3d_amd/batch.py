@@ -174,7 +174,8 @@ class BatchSolver:
         d["active"].fill_(1)
 
     def init_from_macros(self, fields):
-        """Warm start: f = feq(rho, v) per sample from dense (rho (nx,ny,nz), v (nx,ny,nz,3)) pairs."""
+        """Warm start: f = feq(rho, v) per sample from dense (rho (nx,ny,nz), v (nx,ny,nz,3))
+        pairs (numpy arrays or torch tensors)."""
         torch = _torch()
         total = int(self.sizes.cells)
         rho = torch.ones(total, dtype=torch.float32, device=self.device)
@@ -184,8 +185,9 @@ class BatchSolver:
             sm = self.samples[s]
             cb, n = int(sm.cell_begin), int(sm.n_cells)
             idx = cells[cb:cb + n].long()
-            rd = torch.as_tensor(np.asarray(r, np.float32).reshape(-1), device=self.device)
-            vd = torch.as_tensor(np.asarray(v, np.float32).reshape(-1, 3), device=self.device)
+            rd = (r if torch.is_tensor(r) else torch.from_numpy(np.asarray(r))).to(self.device, torch.float32)
+            vd = (v if torch.is_tensor(v) else torch.from_numpy(np.asarray(v))).to(self.device, torch.float32)
+            rd, vd = rd.reshape(-1), vd.reshape(-1, 3)
             rho[cb:cb + n] = rd[idx]
             u[:, cb:cb + n] = vd[idx].t()
         self._init_state(rho, u.contiguous())
@@ -345,19 +347,23 @@ class BatchSolver:
         self._dev["active"].copy_(torch.from_numpy(keep.astype(np.uint8)).to(self.device))
 
     def run_to_steady(self, tol: float = 1e-6, interval: int = 500, max_steps: int = 50_000, graph_steps: int = 0,
-                      callback=None):
+                      callback=None, check_every: int = 0):
         """Step until every sample meets |d<vx>| / |<vx>| < tol between checks
         `interval` steps apart; a sample that meets it is retired at once.
-        callback(t, reduce_rows) is called after every check.
+        Reductions run every `check_every` steps (default `interval`, which it
+        must divide); callback(t, reduce_rows) is called after each.
         Returns (steps per sample, converged mask)."""
-        if interval % 2:
-            raise ValueError("interval must be even (retired samples keep the even-step layout)")
+        check_every = check_every or interval
+        if interval % 2 or check_every % 2 or interval % check_every:
+            raise ValueError("interval and check_every must be even, and check_every must divide interval "
+                             "(retired samples keep the even-step layout)")
+        lag = interval // check_every
         n = self.n_samples
         done = np.zeros(n, bool)
         steps = np.full(n, -1, np.int64)
-        prev = None
+        past = []  # <vx> per sample at the last `lag` checks
         while self._t < max_steps and not done.all():
-            self.run(min(interval, max_steps - self._t), graph_steps=graph_steps)
+            self.run(min(check_every, max_steps - self._t), graph_steps=graph_steps)
             r = self.reduce()
             bad = r[:, 5] > 0
             if bad.any():
@@ -366,13 +372,14 @@ class BatchSolver:
             cur = r[:, 1] / np.maximum(r[:, 6], 1)
             if callback is not None:
                 callback(self._t, r)
-            if prev is not None:
+            if len(past) == lag:
+                prev = past.pop(0)
                 rel = np.abs(cur - prev) / np.maximum(np.abs(cur), 1e-300)
                 newly = (~done) & (rel < tol) & (cur != 0)
                 steps[newly] = self._t
                 done |= newly
                 if newly.any():
                     self.retire(done)
-            prev = cur
+            past.append(cur)
         steps[~done] = self._t
         return steps, done

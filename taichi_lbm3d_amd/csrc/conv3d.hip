@@ -295,8 +295,8 @@ int check_conv(int32_t n, int32_t d, int32_t h, int32_t wd, int32_t cin, int32_t
   if (ks != 1 && ks != 3) return cfail(LBM3D_EINVAL, "kernel size must be 1 or 3");
   if (cin < 16 || cin > 256 || cin % 16 || cout < 16 || cout > 256 || cout % 16)
     return cfail(LBM3D_EINVAL, "channel counts must be multiples of 16 in [16, 256]");
-  if ((int64_t)n * d * h * wd * (cin > cout ? cin : cout) >= ((int64_t)1 << 31))
-    return cfail(LBM3D_ESIZE, "tensor exceeds 2^31 elements");
+  // voxel indices are 32-bit (exact magic division below 2^31); element offsets are 64-bit
+  if ((int64_t)n * d * h * wd >= ((int64_t)1 << 31)) return cfail(LBM3D_ESIZE, "more than 2^31-1 voxels");
   return LBM3D_OK;
 }
 

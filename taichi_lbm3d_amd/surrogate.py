@@ -30,6 +30,7 @@ HEAD_PAD = 16       # the 4 output channels are computed as 16 (12 zero rows)
 DIST_SCALE = 8.0
 RHO_SCALE = 100.0
 CHANNELS = (16, 32, 64, 128)
+BORDER = 16         # cells next to a volume face that the loss ignores and inference pads around
 
 
 def _torch():
@@ -220,24 +221,30 @@ class UNet3D:
 
 # ---- inputs and targets -----------------------------------------------------------------------------
 
-def periodic_distance(solid: np.ndarray, margin: int = 24) -> np.ndarray:
+def periodic_distance(solid, margin: int = 24, device=None):
     """Euclidean distance of every cell to the nearest solid cell under periodic
-    wrap (exact up to `margin` cells; larger distances are capped there)."""
-    from scipy import ndimage
-    s = np.asarray(solid) != 0
-    if not s.any():
-        return np.full(s.shape, float(margin), np.float32)
-    m = min(margin, *s.shape)
-    padded = np.pad(~s, m, mode="wrap")
-    d = ndimage.distance_transform_edt(padded)
-    d = d[m:-m, m:-m, m:-m]
-    return np.minimum(d, margin).astype(np.float32)
+    wrap, on the GPU: the separable squared-distance transform, one min-plus pass
+    per axis over offsets |d| <= margin (exact up to `margin` cells; larger
+    distances are capped there).  Returns a float32 tensor on the device."""
+    torch = _torch()
+    s = torch.as_tensor(np.asarray(solid) != 0 if not torch.is_tensor(solid) else solid != 0, device=device)
+    big = float(3 * margin * margin + 1)
+    f = torch.where(s, torch.zeros((), device=s.device), torch.full((), big, device=s.device))
+    for axis in range(3):
+        g = f.clone()
+        for d in range(1, margin + 1):
+            dd = float(d * d)
+            g = torch.minimum(g, torch.roll(f, d, axis) + dd)
+            g = torch.minimum(g, torch.roll(f, -d, axis) + dd)
+        f = g
+    return torch.sqrt(f).clamp_(max=float(margin))
 
 
-def make_inputs(solid: np.ndarray):
-    """(X, Y, Z, 2) float32: solid mask and distance / DIST_SCALE."""
-    s = (np.asarray(solid) != 0).astype(np.float32)
-    return np.stack([s, periodic_distance(solid) / DIST_SCALE], axis=-1)
+def make_inputs(solid, device=None):
+    """(X, Y, Z, 2) float32 tensor on the device: solid mask and distance / DIST_SCALE."""
+    torch = _torch()
+    s = torch.as_tensor(np.asarray(solid) != 0, device=device)
+    return torch.stack([s.float(), periodic_distance(s, device=device) / DIST_SCALE], dim=-1)
 
 
 def make_targets(rho, v, solid, fx, nu_eff):
@@ -248,27 +255,62 @@ def make_targets(rho, v, solid, fx, nu_eff):
     return np.where(fluid, np.concatenate([vt, rt], axis=-1), 0.0).astype(np.float32)
 
 
-def fields_from_prediction(pred, solid, fx, nu_eff):
-    """Inverse of make_targets: (rho, v) dense arrays from a (X,Y,Z,4) prediction."""
-    p = np.asarray(pred, np.float32)
-    fluid = (np.asarray(solid) == 0)[..., None]
-    v = np.where(fluid, p[..., :3] * (fx / nu_eff), 0.0).astype(np.float32)
-    rho = (1.0 + p[..., 3] * fx * RHO_SCALE).astype(np.float32)
-    return rho, v
+def fields_from_prediction(pred, fluid, fx, nu_eff):
+    """Inverse of make_targets on the device: (rho, v, predicted rho) from a
+    (X,Y,Z,4) prediction, zero velocity off the fluid.
 
-
-def to_network_input(inp, device):
-    """(X,Y,Z,2) float32 -> (1,X,Y,Z,16) bf16 on the device (channels 2..15 zero)."""
+    The warm start takes the predicted velocity with rho = 1, the cold start's
+    density.  Every isolated pore keeps its initial mass for good, and the
+    steady velocity of a pore scales as 1/rho, so a predicted density would pin
+    each pore's mean density -- and the permeability, to about the prediction's
+    density error -- away from the cold start's.  The predicted density is
+    returned for error reporting only."""
     torch = _torch()
-    t = torch.zeros((1,) + inp.shape[:3] + (CIN_PAD,), dtype=torch.bfloat16, device=device)
-    t[..., :2] = torch.from_numpy(np.ascontiguousarray(inp)).to(device).to(torch.bfloat16)
+    p = pred.float()
+    fl = fluid.to(p.dtype)
+    v = p[..., :3] * (fx / nu_eff) * fl[..., None]
+    rho_pred = 1.0 + p[..., 3] * (fx * RHO_SCALE) * fl
+    return torch.ones_like(rho_pred), v, rho_pred
+
+
+def to_network_input(inp):
+    """(N,X,Y,Z,2) float32 tensor -> (N,X,Y,Z,16) bf16 (channels 2..15 zero)."""
+    torch = _torch()
+    t = torch.zeros(inp.shape[:4] + (CIN_PAD,), dtype=torch.bfloat16, device=inp.device)
+    t[..., :2] = inp.to(torch.bfloat16)
     return t
 
 
-def predict(net, solid, fx, nu_eff):
-    """Steady (rho, v) predicted for one periodic geometry (extents multiples of 8)."""
+def periodic_pad(t, pad):
+    """(X,Y,Z,...) tensor -> (X+2p, Y+2p, Z+2p, ...), wrapped periodically."""
     torch = _torch()
+    for axis in range(3):
+        n = t.shape[axis]
+        idx = (torch.arange(-pad, n + pad, device=t.device) % n)
+        t = t.index_select(axis, idx)
+    return t
+
+
+def predict(net, solids, fx, nu_eff, batch: int = 2):
+    """Warm-start (rho = 1, v) device tensors predicted for periodic geometries of
+    one shape, `batch` volumes per forward pass.  Each volume is padded
+    periodically by at least BORDER cells (to a multiple of 8) so every cell is
+    predicted with its true surroundings; the centre is kept."""
+    torch = _torch()
+    out = []
     with torch.no_grad():
-        x = to_network_input(make_inputs(solid), net.device)
-        out = net(x)[0].cpu().numpy()
-    return fields_from_prediction(out, solid, fx, nu_eff)
+        for b0 in range(0, len(solids), batch):
+            group = solids[b0:b0 + batch]
+            shape = np.asarray(group[0]).shape
+            pads = [BORDER + (-(n + 2 * BORDER)) % 8 // 2 for n in shape]
+            pad = max(pads)
+            while any((n + 2 * pad) % 8 for n in shape):
+                pad += 1
+            inp = torch.stack([periodic_pad(make_inputs(s, device=net.device), pad) for s in group])
+            pred = net(to_network_input(inp))
+            X, Y, Z = shape
+            for i, s in enumerate(group):
+                fluid = torch.as_tensor(np.asarray(s) == 0, device=net.device)
+                core = pred[i, pad:pad + X, pad:pad + Y, pad:pad + Z]
+                out.append(fields_from_prediction(core, fluid, fx, nu_eff)[:2])
+    return out

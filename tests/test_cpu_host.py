@@ -390,3 +390,14 @@ def test_winding_number_oracle_closed_meshes():
     w = winding_number(sph, pts)
     far = np.abs(r - 2.0) > 0.05
     np.testing.assert_allclose(w[far], (r[far] < 2.0).astype(float), atol=1e-9)
+
+
+def test_surrogate_distance_input_matches_scipy_edt():
+    """The surrogate's distance channel (separable min-plus transform in torch,
+    run here on the CPU) equals scipy's exact EDT on a wrap-padded copy."""
+    from oracle.geo_ref import periodic_distance_edt
+    from taichi_lbm3d_amd.geometry import sphere_pack
+    from taichi_lbm3d_amd.surrogate import periodic_distance
+    for L, por in [(24, 0.3), (40, 0.6), (48, 0.9)]:
+        s = sphere_pack((L, L, L), porosity=por, rmin=3, rmax=6, seed=L, periodic=(True,) * 3)
+        np.testing.assert_array_equal(periodic_distance(s).numpy(), periodic_distance_edt(s))
