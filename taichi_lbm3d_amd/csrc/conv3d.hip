@@ -25,6 +25,10 @@
 #include "lbm3d.h"
 #include "lbm3d_conv.h"
 
+#ifndef LBM3D_CONV_MW
+#define LBM3D_CONV_MW 2  // 16-voxel fragments per wave in the forward kernel
+#endif
+
 namespace {
 
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
@@ -122,60 +126,95 @@ __device__ __forceinline__ uint16_t to_bf16(float f) {
 // MFMA 16x16x32 lane layout: A (16 x 32): lane l holds row l%16, k = 8*(l/16) .. +7;
 // B (32 x 16): column l%16, the same k; C (16 x 16): column l%16, rows 4*(l/16) .. +3.
 // ---------------------------------------------------------------------------
-template <int NT>
+// Each wave computes MW fragments of 16 voxels x NT fragments of 16 output
+// channels; the next K chunk's operands are loaded while the current chunk's
+// MFMAs run.
+template <int NT, int MW>
 __global__ void __launch_bounds__(256)
     k_conv_fwd(const uint16_t *__restrict__ x, const uint16_t *__restrict__ w, const float *__restrict__ bias,
                uint16_t *__restrict__ y, const Vol vol, int cin, int cout, int ks, int kpad, int relu) {
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int g = lane >> 4, r16 = lane & 15;
-  const uint32_t m0 = blockIdx.x * 64u + (uint32_t)wave * 16u;
+  const uint32_t mb = blockIdx.x * (64u * MW) + (uint32_t)wave * (16u * MW);
   const int co0 = blockIdx.y * 16 * NT;
-  const uint32_t m = m0 + (uint32_t)r16;
-  const bool mvalid = m < vol.vox;
-  int vb = 0, vz = 0, vy = 0, vx = 0;
-  if (mvalid) decode(vol, m, vb, vz, vy, vx);
-  const int K = ks * ks * ks * cin;
-  int kk = 8 * g;
-  int tap = kk / cin, ci = kk - tap * cin;
-  f32x4 acc[NT];
+  bool mvalid[MW];
+  int vb[MW], vz[MW], vy[MW], vx[MW];
 #pragma unroll
-  for (int t = 0; t < NT; ++t) acc[t] = f32x4{0.f, 0.f, 0.f, 0.f};
+  for (int f = 0; f < MW; ++f) {
+    const uint32_t m = mb + 16u * f + (uint32_t)r16;
+    mvalid[f] = m < vol.vox;
+    vb[f] = vz[f] = vy[f] = vx[f] = 0;
+    if (mvalid[f]) decode(vol, m, vb[f], vz[f], vy[f], vx[f]);
+  }
+  const int K = ks * ks * ks * cin;
   const uint16_t *wrow[NT];
 #pragma unroll
-  for (int t = 0; t < NT; ++t) wrow[t] = w + (size_t)(co0 + 16 * t + r16) * kpad;
-  for (int kc = 0; kc < kpad; kc += 32) {
+  for (int t = 0; t < NT; ++t) wrow[t] = w + (size_t)(co0 + 16 * t + r16) * kpad + 8 * g;
+  f32x4 acc[MW][NT];
+#pragma unroll
+  for (int f = 0; f < MW; ++f)
+#pragma unroll
+    for (int t = 0; t < NT; ++t) acc[f][t] = f32x4{0.f, 0.f, 0.f, 0.f};
+
+  // this lane's 8 K values of a chunk: k = kc + 8g .. +7, i.e. channels ci .. ci+7 of tap
+  int kk = 8 * g;
+  int tap = kk / cin, ci = kk - tap * cin;
+  auto load_a = [&](int f, int tp, int c, int k) -> u32x4 {
     u32x4 a = u32x4{0u, 0u, 0u, 0u};
-    if (mvalid && kk < K) {
+    if (mvalid[f] && k < K) {
       int dz, dy, dx;
-      tap_offset(tap, ks, dz, dy, dx);
-      const int64_t src = shifted(vol, vb, vz, vy, vx, dz, dy, dx);
-      if (src >= 0) a = ld16(x + src * cin + ci);
+      tap_offset(tp, ks, dz, dy, dx);
+      const int64_t src = shifted(vol, vb[f], vz[f], vy[f], vx[f], dz, dy, dx);
+      if (src >= 0) a = ld16(x + src * cin + c);
     }
-    const bf16x8 av = __builtin_bit_cast(bf16x8, a);
+    return a;
+  };
+  u32x4 a[MW], b[NT];
+#pragma unroll
+  for (int f = 0; f < MW; ++f) a[f] = load_a(f, tap, ci, kk);
+#pragma unroll
+  for (int t = 0; t < NT; ++t) b[t] = ld16(wrow[t]);
+  for (int kc = 0; kc < kpad; kc += 32) {
+    int kk2 = kk + 32, ci2 = ci + 32, tap2 = tap;
+    while (ci2 >= cin) {
+      ci2 -= cin;
+      ++tap2;
+    }
+    u32x4 an[MW], bn[NT];
+    const bool more = kc + 32 < kpad;
+#pragma unroll
+    for (int f = 0; f < MW; ++f) an[f] = more ? load_a(f, tap2, ci2, kk2) : u32x4{0u, 0u, 0u, 0u};
+#pragma unroll
+    for (int t = 0; t < NT; ++t) bn[t] = more ? ld16(wrow[t] + kc + 32) : u32x4{0u, 0u, 0u, 0u};
 #pragma unroll
     for (int t = 0; t < NT; ++t) {
-      const bf16x8 bv = __builtin_bit_cast(bf16x8, ld16(wrow[t] + kc + 8 * g));
-      acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, bv, acc[t], 0, 0, 0);
+      const bf16x8 bv = __builtin_bit_cast(bf16x8, b[t]);
+#pragma unroll
+      for (int f = 0; f < MW; ++f)
+        acc[f][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, a[f]), bv, acc[f][t], 0, 0, 0);
     }
-    kk += 32;
-    ci += 32;
-    while (ci >= cin) {
-      ci -= cin;
-      ++tap;
-    }
+#pragma unroll
+    for (int f = 0; f < MW; ++f) a[f] = an[f];
+#pragma unroll
+    for (int t = 0; t < NT; ++t) b[t] = bn[t];
+    kk = kk2;
+    ci = ci2;
+    tap = tap2;
   }
 #pragma unroll
   for (int t = 0; t < NT; ++t) {
     const int co = co0 + 16 * t + r16;
     const float bb = bias ? bias[co] : 0.f;
 #pragma unroll
-    for (int e = 0; e < 4; ++e) {
-      const uint32_t mo = m0 + (uint32_t)(4 * g + e);
-      if (mo >= vol.vox) continue;
-      float val = acc[t][e] + bb;
-      if (relu) val = val > 0.f ? val : 0.f;
-      y[(size_t)mo * cout + co] = to_bf16(val);
-    }
+    for (int f = 0; f < MW; ++f)
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        const uint32_t mo = mb + 16u * f + (uint32_t)(4 * g + e);
+        if (mo >= vol.vox) continue;
+        float val = acc[f][t][e] + bb;
+        if (relu) val = val > 0.f ? val : 0.f;
+        y[(size_t)mo * cout + co] = to_bf16(val);
+      }
   }
 }
 
@@ -337,13 +376,14 @@ int lbm3d_conv3d_fwd(void *stream, const uint16_t *x, const uint16_t *w, const f
   const Vol vol = make_vol(n, d, h, wd);
   const int kpad = lbm3d_conv_kpad(cin, ks);
   hipStream_t s = (hipStream_t)stream;
-  const unsigned gx = (vol.vox + 63u) / 64u;
+  constexpr int MW = LBM3D_CONV_MW;
+  const unsigned gx = (vol.vox + 64u * MW - 1u) / (64u * MW);
   if (cout % 64 == 0)
-    k_conv_fwd<4><<<dim3(gx, cout / 64), 256, 0, s>>>(x, w, bias, y, vol, cin, cout, ks, kpad, relu);
+    k_conv_fwd<4, MW><<<dim3(gx, cout / 64), 256, 0, s>>>(x, w, bias, y, vol, cin, cout, ks, kpad, relu);
   else if (cout % 32 == 0)
-    k_conv_fwd<2><<<dim3(gx, cout / 32), 256, 0, s>>>(x, w, bias, y, vol, cin, cout, ks, kpad, relu);
+    k_conv_fwd<2, MW><<<dim3(gx, cout / 32), 256, 0, s>>>(x, w, bias, y, vol, cin, cout, ks, kpad, relu);
   else
-    k_conv_fwd<1><<<dim3(gx, cout / 16), 256, 0, s>>>(x, w, bias, y, vol, cin, cout, ks, kpad, relu);
+    k_conv_fwd<1, MW><<<dim3(gx, cout / 16), 256, 0, s>>>(x, w, bias, y, vol, cin, cout, ks, kpad, relu);
   return ccheck("lbm3d_conv3d_fwd");
 }
 

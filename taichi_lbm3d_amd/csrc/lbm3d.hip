@@ -35,10 +35,6 @@ constexpr int kQ = 19;
 #endif
 constexpr int kBlock = LBM3D_BLOCK;  // threads per block, a multiple of 64
 constexpr int kWinWords = 384;       // LDS row window: 3 planes x (J+3) rows x W words
-#ifndef LBM3D_STAGE_LEN
-#define LBM3D_STAGE_LEN 320
-#endif
-constexpr int kStage = LBM3D_STAGE_LEN;  // LDS-staged elements per direction (odd steps)
 
 // Build-time knobs (defaults are the measured best on MI355X).
 #ifndef LBM3D_XCD_REMAP_EVEN
@@ -58,9 +54,6 @@ constexpr int kStage = LBM3D_STAGE_LEN;  // LDS-staged elements per direction (o
 #endif
 #ifndef LBM3D_ROW_WINDOW
 #define LBM3D_ROW_WINDOW 1    // odd steps read row words from a per-workgroup LDS window
-#endif
-#ifndef LBM3D_STAGE_ODD
-#define LBM3D_STAGE_ODD 0     // odd steps gather and scatter through LDS-staged ranges
 #endif
 #ifndef LBM3D_CPOL_LOAD
 #define LBM3D_CPOL_LOAD 0     // cache-policy bits of population loads
@@ -832,143 +825,6 @@ __global__ void __launch_bounds__(kBlock, LBM3D_STEP_MINB)
   step_cell<T, ODD, FORCED>(kp, kp.geo, sl, rv, cells, face_bits, face_u, x, first);
 }
 
-// Odd step with the scattered accesses staged through LDS.  For each direction
-// q the neighbour ids a workgroup's cells address form a short contiguous range
-// [lo_q, hi_q]; the workgroup loads that range of slot q with coalesced loads,
-// its cells read their populations from LDS, write post-collision values back
-// there, and the workgroup stores the elements its cells wrote (every slot
-// element has exactly one writer, so the others -- owned by neighbouring
-// workgroups -- are skipped).  Ranges longer than kStage fall back to direct
-// accesses.  Written elements are told apart by a sentinel bit pattern; a
-// computed value with that pattern is stored directly instead.
-__device__ __forceinline__ int wave_min_i32(int v) {
-  for (int o = 32; o > 0; o >>= 1) {
-    const int t = __shfl_xor(v, o, 64);
-    v = t < v ? t : v;
-  }
-  return v;
-}
-__device__ __forceinline__ int wave_max_i32(int v) {
-  for (int o = 32; o > 0; o >>= 1) {
-    const int t = __shfl_xor(v, o, 64);
-    v = t > v ? t : v;
-  }
-  return v;
-}
-
-constexpr uint32_t kUnwritten = 0xFFFFFFFFu;
-
-template <typename T, bool FORCED>
-__global__ void __launch_bounds__(kBlock)
-    k_step_odd_staged(const KP kp, T *__restrict__ g, const int32_t *__restrict__ cells,
-                      const uint4 *__restrict__ rows, const uint32_t *__restrict__ face_bits,
-                      typename St<T>::R *__restrict__ face_u, int first) {
-  using R = typename St<T>::R;
-  static_assert(sizeof(R) == 4, "LDS staging holds 32-bit values");
-  __shared__ uint4 win[kWinWords];
-  __shared__ uint32_t stage[18 * kStage];
-  __shared__ int s_lo[18], s_hi[18];
-  const Geo &gm = kp.geo;
-  const uint32_t x0 = blockIdx.x * kBlock;
-  const uint32_t n = (uint32_t)kp.p.n_cells;
-  if (x0 >= n) return;
-  const uint32_t x = x0 + threadIdx.x;
-  const bool valid = x < n;
-  const Slots<T> sl(g, kp.S);
-  const Rows rb(rows, gm.n_row_words);
-  const RowView rv = row_window(gm, cells, rows, rb, x0, n, win);
-  if (threadIdx.x < 18) {
-    s_lo[threadIdx.x] = INT32_MAX;
-    s_hi[threadIdx.x] = -1;
-  }
-  __syncthreads();
-  CellCtx cc;
-  int32_t nb[18];
-  if (valid) {
-    coords(gm, (uint32_t)cells[x], cc.i, cc.j, cc.k);
-    cc.have_coords = true;
-    if (rv.win) {
-      const WindowRows src{rv.win, gm.W, rv.JW, cc.j - rv.j0};
-      neighbor_ids(src, gm, cc.k, nb);
-    } else {
-      const TableRows src(*rv.rb, gm, cc.i, cc.j);
-      neighbor_ids(src, gm, cc.k, nb);
-    }
-  } else {
-#pragma unroll
-    for (int q = 0; q < 18; ++q) nb[q] = -1;
-  }
-  const int lane = threadIdx.x & 63;
-#pragma unroll
-  for (int q = 0; q < 18; ++q) {
-    const int lo = wave_min_i32(nb[q] >= 0 ? nb[q] : INT32_MAX);
-    const int hi = wave_max_i32(nb[q]);
-    if (lane == 0) {
-      atomicMin(&s_lo[q], lo);
-      atomicMax(&s_hi[q], hi);
-    }
-  }
-  __syncthreads();
-  // coalesced loads of the staged ranges
-  for (int q = 0; q < 18; ++q) {
-    const int lo = s_lo[q], hi = s_hi[q];
-    if (hi < 0 || hi - lo + 1 > kStage) continue;
-    for (int e = threadIdx.x; e <= hi - lo; e += kBlock)
-      stage[q * kStage + e] = __float_as_uint(sl.ld_slot(q + 1, (uint32_t)(lo + e) * (uint32_t)sizeof(T)));
-  }
-  __syncthreads();
-  R v[kQ];
-  const uint32_t xb = x * (uint32_t)sizeof(T);
-  if (valid) {
-    v[0] = sl.ld_slot(0, xb);
-#pragma unroll
-    for (int q = 1; q < kQ; ++q) {
-      const int32_t c = nb[q - 1];
-      const int lo = s_lo[q - 1], hi = s_hi[q - 1];
-      const bool staged = hi >= 0 && hi - lo + 1 <= kStage;
-      if (c >= 0)
-        v[opp(q)] = staged ? __uint_as_float(stage[(q - 1) * kStage + (c - lo)])
-                           : sl.ld_slot(q, (uint32_t)c * (uint32_t)sizeof(T));
-      else
-        v[opp(q)] = sl.ld_slot(opp(q), xb);  // bounce-back: the cell's own slot
-    }
-  }
-  __syncthreads();
-  for (int q = 0; q < 18; ++q) {
-    const int lo = s_lo[q], hi = s_hi[q];
-    if (hi < 0 || hi - lo + 1 > kStage) continue;
-    for (int e = threadIdx.x; e <= hi - lo; e += kBlock) stage[q * kStage + e] = kUnwritten;
-  }
-  __syncthreads();
-  if (valid) {
-    collide_cell<R, FORCED>(kp, gm, cells, face_bits, face_u, x, first, v, cc);
-    sl.st_slot(0, xb, v[0]);
-#pragma unroll
-    for (int q = 1; q < kQ; ++q) {
-      const int32_t c = nb[q - 1];
-      const int lo = s_lo[q - 1], hi = s_hi[q - 1];
-      const bool staged = hi >= 0 && hi - lo + 1 <= kStage;
-      if (c >= 0) {
-        if (staged && __float_as_uint(v[q]) != kUnwritten)
-          stage[(q - 1) * kStage + (c - lo)] = __float_as_uint(v[q]);
-        else
-          sl.st_slot(q, (uint32_t)c * (uint32_t)sizeof(T), v[q]);
-      } else {
-        sl.st_slot(opp(q), xb, v[q]);
-      }
-    }
-  }
-  __syncthreads();
-  for (int q = 0; q < 18; ++q) {
-    const int lo = s_lo[q], hi = s_hi[q];
-    if (hi < 0 || hi - lo + 1 > kStage) continue;
-    for (int e = threadIdx.x; e <= hi - lo; e += kBlock) {
-      const uint32_t w = stage[q * kStage + e];
-      if (w != kUnwritten) sl.st_slot(q + 1, (uint32_t)(lo + e) * (uint32_t)sizeof(T), __uint_as_float(w));
-    }
-  }
-}
-
 template <typename T, bool ODD>
 __global__ void __launch_bounds__(kBlock)
     k_macros(const KP kp, const T *__restrict__ g, const int32_t *__restrict__ cells,
@@ -1409,13 +1265,6 @@ void launch_step_parity(hipStream_t s, const KP &kp, void *g, const int32_t *cel
                         const uint32_t *face_bits, void *face_u, int64_t t) {
   using R = typename St<T>::R;
   const unsigned grid = grid_for(kp.p.n_cells);
-  if constexpr (LBM3D_STAGE_ODD && sizeof(R) == 4) {
-    if (t & 1) {
-      k_step_odd_staged<T, FORCED><<<(unsigned)((kp.p.n_cells + kBlock - 1) / kBlock), kBlock, 0, s>>>(
-          kp, (T *)g, cells, rows, face_bits, (R *)face_u, t == 0);
-      return;
-    }
-  }
   if (t & 1)
     k_step<T, true, FORCED><<<grid, kBlock, 0, s>>>(kp, (T *)g, cells, rows, face_bits, (R *)face_u, t == 0);
   else
