@@ -24,7 +24,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0           # MI355X HBM3E, GB/s
-BYTES_PER_FLUP = {"f64": 304, "f32": 152, "f16": 76, "bf16": 76}  # 19 reads + 19 writes per fluid-cell update
+BYTES_PER_FLUP = {"f64": 304, "f32": 152, "f16": 76, "bf16": 76, "f16sr": 76, "bf16sr": 76}  # 19 reads + 19 writes per fluid-cell update
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "step_traffic_pack256.json")
 
 
@@ -96,7 +96,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=200)
     ap.add_argument("--n", type=int, default=256)
     ap.add_argument("--seed", type=int, default=0)
-    ap.add_argument("--dtype", default="f32", choices=["f32", "f16", "bf16", "f64"])
+    ap.add_argument("--dtype", default="f32", choices=["f32", "f16", "bf16", "f64", "f16sr", "bf16sr"])
     ap.add_argument("--no-steady", action="store_true")
     ap.add_argument("--max-steady-steps", type=int, default=50000)
     ap.add_argument("--no-cpu-baseline", action="store_true")

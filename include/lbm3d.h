@@ -39,13 +39,21 @@
 extern "C" {
 #endif
 
-#define LBM3D_ABI_VERSION 6
+#define LBM3D_ABI_VERSION 7
 
 /* storage types for populations; F32, F16 and BF16 compute in FP32, F64 in FP64 */
 #define LBM3D_F32 0
 #define LBM3D_F16 1
 #define LBM3D_BF16 2
 #define LBM3D_F64 3
+
+/* lbm3d_params.flags: F16/BF16 population stores of the step kernels round
+ * stochastically (unbiased: each stored value equals the exact one in
+ * expectation) instead of to nearest, with random bits that depend only on
+ * the step, the slot and the element, so a run is reproducible.  It keeps
+ * per-step increments far below the 16-bit spacing, e.g. of a weak body
+ * force, from rounding away.  No effect on F32/F64 storage. */
+#define LBM3D_FLAG_STOCHASTIC_ROUNDING 1
 
 /* error codes */
 #define LBM3D_OK 0
@@ -68,7 +76,7 @@ typedef struct lbm3d_params {
   int32_t force_on;        /* force_flag (SP:137-140) */
   int32_t bc_mode[6];      /* 0 periodic, 1 fixed pressure, 2 fixed velocity (SP:22) */
   int32_t face_u_off[6];   /* element offset of each pressure face's state in face_u, -1 if unused */
-  int32_t pad_;            /* keeps the doubles 8-byte aligned */
+  int32_t flags;           /* LBM3D_FLAG_* bits (0: none) */
   double s_v, s_o;         /* relaxation rates s_v and s_other (SP:127-129) */
   double fx, fy, fz;       /* body force (SP:17, SP:134-136) */
   double guo_a, guo_b;     /* force-term scalings: SP uses 1/3 and 1/9 (SP:236) */

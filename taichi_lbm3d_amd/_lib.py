@@ -15,9 +15,18 @@ LIB_PATH = os.path.join(_HERE, "liblbm3d.so")
 SOURCES = [os.path.join(_HERE, "csrc", name) for name in ("lbm3d.hip", "conv3d.hip", "geo_io.hip")]
 INCLUDE = os.path.join(_ROOT, "include")
 
-ABI_VERSION = 6
+ABI_VERSION = 7
 F32, F16, BF16, F64 = 0, 1, 2, 3
-DTYPES = {"f32": F32, "f16": F16, "bf16": BF16, "f64": F64}
+FLAG_STOCHASTIC_ROUNDING = 1
+# storage name -> storage type code; "...sr" names are the 16-bit types with
+# stochastic rounding (the flag bits of STORAGE_FLAGS set in lbm3d_params.flags)
+DTYPES = {"f32": F32, "f16": F16, "bf16": BF16, "f64": F64, "f16sr": F16, "bf16sr": BF16}
+STORAGE_FLAGS = {"f16sr": FLAG_STOCHASTIC_ROUNDING, "bf16sr": FLAG_STOCHASTIC_ROUNDING}
+
+
+def torch_storage_dtype(name: str):
+    import torch
+    return {F32: torch.float32, F16: torch.float16, BF16: torch.bfloat16, F64: torch.float64}[DTYPES[name]]
 HIPCC_FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",
     # keep FP32 arithmetic unpacked (no v_pk_*_f32)
@@ -49,7 +58,7 @@ class Params(ctypes.Structure):
         ("n_cells", ctypes.c_int32), ("force_on", ctypes.c_int32),
         ("bc_mode", ctypes.c_int32 * 6),
         ("face_u_off", ctypes.c_int32 * 6),
-        ("pad_", ctypes.c_int32),
+        ("flags", ctypes.c_int32),
         ("s_v", ctypes.c_double), ("s_o", ctypes.c_double),
         ("fx", ctypes.c_double), ("fy", ctypes.c_double), ("fz", ctypes.c_double),
         ("guo_a", ctypes.c_double), ("guo_b", ctypes.c_double),

@@ -72,6 +72,7 @@ class BatchSolver:
         p.guo_a, p.guo_b = CONVENTIONS[self.convention]
         p.fx, p.fy, p.fz = self.fx, self.fy, self.fz
         p.force_on = 1 if (self.fx != 0 or self.fy != 0 or self.fz != 0) else 0
+        p.flags = _lib.STORAGE_FLAGS.get(self.dtype, 0)
         for f, name in enumerate(FACE_NAMES):
             p.face_u_off[f] = -1
             p.rho_bc[f] = 1.0
@@ -148,8 +149,7 @@ class BatchSolver:
             lib.call("lbm3d_batch_desc", _ref(self.p), self.samples, n, ctypes_void(host_desc.ctypes.data),
                      int(sizes.desc_bytes))
             desc = torch.from_numpy(host_desc).to(dev)
-            tdt = {"f32": torch.float32, "f16": torch.float16, "bf16": torch.bfloat16,
-                   "f64": torch.float64}[self.dtype]
+            tdt = _lib.torch_storage_dtype(self.dtype)
             stride = lib.size("lbm3d_slot_stride", total)
             g = torch.zeros(19 * stride, dtype=tdt, device=dev)
             face_u = torch.zeros(max(int(sizes.face_u), 1), dtype=self._real_dtype(), device=dev)

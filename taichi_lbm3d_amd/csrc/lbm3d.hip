@@ -133,6 +133,7 @@ struct KP {
   Phys<double> pd;
   Geo geo;
   uint32_t S;              // slot stride
+  uint32_t sr_seed;        // per-step seed of stochastic rounding, 0 = round to nearest
 };
 
 // Exact division of 31-bit numerators by d: m = ceil(2^(31+l) / d), l = ceil(log2 d).
@@ -183,6 +184,7 @@ KP make_kp(const lbm3d_params &p) {
   fill_phys(k.pd, p);
   k.S = slot_stride((uint32_t)(p.n_cells > 0 ? p.n_cells : 0));
   k.geo = make_geo(p.nx, p.ny, p.nz, p.face_u_off);
+  k.sr_seed = 0u;
   return k;
 }
 
@@ -348,6 +350,7 @@ constexpr int kBufferFlags = 0x00020000;  // dword 3 of a raw buffer resource on
 template <typename T> struct St;
 template <> struct St<float> {
   using R = float;
+  static constexpr bool kStochastic = false;
   static __device__ __forceinline__ float ld(__amdgpu_buffer_rsrc_t r, uint32_t vo, uint32_t so) {
     return __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(r, (int)vo, (int)so, LBM3D_CPOL_LOAD));
   }
@@ -355,8 +358,36 @@ template <> struct St<float> {
     __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(v), r, (int)vo, (int)so, LBM3D_CPOL_STORE);
   }
 };
+// Stochastic rounding to a 16-bit format: add uniform random bits to the FP32
+// pattern below the target format's last place, then truncate toward zero.
+// The stored value equals the exact one in expectation, so an increment far
+// below the format's spacing is kept on average instead of always rounding
+// away.  rnd: 32 random bits.
+__device__ __forceinline__ unsigned short bf16_sr(float v, uint32_t rnd) {
+  const uint32_t bits = __float_as_uint(v);
+  if ((bits & 0x7F800000u) == 0x7F800000u) return (unsigned short)(bits >> 16);  // inf / nan
+  return (unsigned short)((bits + (rnd & 0xFFFFu)) >> 16);
+}
+
+__device__ __forceinline__ unsigned short half_sr(float v, uint32_t rnd) {
+  const uint32_t bits = __float_as_uint(v);
+  const uint32_t ex = (bits >> 23) & 0xFFu;
+  if (ex == 0xFFu) return __half_as_ushort(__float2half_rn(v));  // inf / nan
+  // float mantissa bits below the half's last place: 13 for half normals (ex >= 113),
+  // 126 - ex for half subnormals (last place 2^-24)
+  const uint32_t drop = ex >= 113u ? 13u : 126u - ex;
+  if (drop >= 24u) {  // |v| < 2^-24: 0 or the smallest subnormal, with probability |v| / 2^-24
+    const float p = fabsf(v) * 0x1p24f;
+    const unsigned short sign = (unsigned short)((bits >> 16) & 0x8000u);
+    return (float)(rnd >> 8) * 0x1p-24f < p ? (unsigned short)(sign | 1u) : sign;
+  }
+  const uint32_t d = bits + (rnd & ((1u << drop) - 1u));
+  return __half_as_ushort(__float2half_rz(__uint_as_float(d)));
+}
+
 template <> struct St<__half> {
   using R = float;
+  static constexpr bool kStochastic = true;
   static __device__ __forceinline__ float ld(__amdgpu_buffer_rsrc_t r, uint32_t vo, uint32_t so) {
     const unsigned short b = __builtin_amdgcn_raw_buffer_load_b16(r, (int)vo, (int)so, LBM3D_CPOL_LOAD);
     return __half2float(__ushort_as_half(b));
@@ -365,9 +396,20 @@ template <> struct St<__half> {
     __builtin_amdgcn_raw_buffer_store_b16(__half_as_ushort(__float2half_rn(v)), r, (int)vo, (int)so,
                                           LBM3D_CPOL_STORE);
   }
+  static __device__ __forceinline__ void st_sr(__amdgpu_buffer_rsrc_t r, uint32_t vo, uint32_t so, float v,
+                                               uint32_t rnd) {
+    __builtin_amdgcn_raw_buffer_store_b16(half_sr(v, rnd), r, (int)vo, (int)so, LBM3D_CPOL_STORE);
+  }
 };
 template <> struct St<__hip_bfloat16> {
   using R = float;
+  static constexpr bool kStochastic = true;
+  static __device__ __forceinline__ void st_sr(__amdgpu_buffer_rsrc_t r, uint32_t vo, uint32_t so, float v,
+                                               uint32_t rnd) {
+    __builtin_amdgcn_raw_buffer_store_b16(bf16_sr(v, rnd), r, (int)vo, (int)so, LBM3D_CPOL_STORE);
+  }
+    __builtin_amdgcn_raw_buffer_store_b16(b, r, (int)vo, (int)so, LBM3D_CPOL_STORE);
+  }
   static __device__ __forceinline__ float ld(__amdgpu_buffer_rsrc_t r, uint32_t vo, uint32_t so) {
     const unsigned short b = __builtin_amdgcn_raw_buffer_load_b16(r, (int)vo, (int)so, LBM3D_CPOL_LOAD);
     return __uint_as_float((uint32_t)b << 16);
@@ -380,6 +422,7 @@ template <> struct St<__hip_bfloat16> {
 };
 template <> struct St<double> {
   using R = double;
+  static constexpr bool kStochastic = false;
   static __device__ __forceinline__ double ld(__amdgpu_buffer_rsrc_t r, uint32_t vo, uint32_t so) {
     const auto v = __builtin_amdgcn_raw_buffer_load_b64(r, (int)vo, (int)so, LBM3D_CPOL_LOAD);
     return __hiloint2double((int)v[1], (int)v[0]);
@@ -394,12 +437,24 @@ template <> struct St<double> {
 };
 
 // Resource k covers slots 2k-1 and 2k (k = 1..9); resource 0 covers slot 0.
+// 32-bit mixing function (a "lowbias32" integer hash)
+__device__ __forceinline__ uint32_t mix32(uint32_t h) {
+  h ^= h >> 16;
+  h *= 0x7feb352du;
+  h ^= h >> 15;
+  h *= 0x846ca68bu;
+  h ^= h >> 16;
+  return h;
+}
+
 template <typename T>
 struct Slots {
   using R = typename St<T>::R;
   __amdgpu_buffer_rsrc_t r[10];
-  uint32_t Sb;  // slot stride in bytes
-  __device__ __forceinline__ Slots(const T *g, uint32_t S) {
+  uint32_t Sb;    // slot stride in bytes
+  uint32_t seed;  // nonzero: stochastic rounding of 16-bit stores, random bits from (seed, address)
+  __device__ __forceinline__ Slots(const T *g, uint32_t S, uint32_t sr_seed = 0u) {
+    seed = sr_seed;
     Sb = S * (uint32_t)sizeof(T);
     char *base = (char *)const_cast<T *>(g);
     r[0] = __builtin_amdgcn_make_buffer_rsrc(base, (short)0, (int)Sb, kBufferFlags);
@@ -412,12 +467,21 @@ struct Slots {
   __device__ __forceinline__ R ld_slot(int q, uint32_t vo) const {
     return St<T>::ld(r[(q + 1) >> 1], vo, (q > 0 && !(q & 1)) ? Sb : 0u);
   }
+  __device__ __forceinline__ void store(int pair, uint32_t vo, uint32_t so, R v) const {
+    if constexpr (St<T>::kStochastic) {
+      if (seed) {
+        St<T>::st_sr(r[pair], vo, so, v, mix32(seed + ((vo + so) * 10u + (uint32_t)pair) * 0x9E3779B9u));
+        return;
+      }
+    }
+    St<T>::st(r[pair], vo, so, v);
+  }
   __device__ __forceinline__ void st_slot(int q, uint32_t vo, R v) const {
-    St<T>::st(r[(q + 1) >> 1], vo, (q > 0 && !(q & 1)) ? Sb : 0u, v);
+    store((q + 1) >> 1, vo, (q > 0 && !(q & 1)) ? Sb : 0u, v);
   }
   // byte offset vo relative to the pair of direction q (vo < 2 Sb)
   __device__ __forceinline__ R ld_pair(int q, uint32_t vo) const { return St<T>::ld(r[(q + 1) >> 1], vo, 0u); }
-  __device__ __forceinline__ void st_pair(int q, uint32_t vo, R v) const { St<T>::st(r[(q + 1) >> 1], vo, 0u, v); }
+  __device__ __forceinline__ void st_pair(int q, uint32_t vo, R v) const { store((q + 1) >> 1, vo, 0u, v); }
 };
 
 // ---------------------------------------------------------------------------
@@ -814,7 +878,7 @@ __global__ void __launch_bounds__(kBlock, LBM3D_STEP_MINB)
   const uint32_t n = (uint32_t)kp.p.n_cells;
   if (x0 >= n) return;  // whole padding blocks of the grid
   const uint32_t x = x0 + threadIdx.x;
-  const Slots<T> sl(g, kp.S);
+  const Slots<T> sl(g, kp.S, kp.sr_seed);
   const Rows rb(rows, kp.geo.n_row_words);
   RowView rv{&rb, nullptr, 0u, 0};
   if constexpr (ODD && LBM3D_ROW_WINDOW) {
@@ -882,7 +946,7 @@ __global__ void __launch_bounds__(B)
     k_step_batch(const KP kp, const Batch bt, T *__restrict__ g, const int32_t *__restrict__ cells,
                  const uint4 *__restrict__ rows, const uint32_t *__restrict__ face_bits,
                  typename St<T>::R *__restrict__ face_u, int first, uint32_t n_tiles, int reverse) {
-  const Slots<T> sl(g, kp.S);
+  const Slots<T> sl(g, kp.S, kp.sr_seed);
   for (uint32_t it = blockIdx.x; it < n_tiles; it += gridDim.x) {
     const uint32_t x0 = (reverse ? n_tiles - 1u - it : it) * (uint32_t)B;
     const int s = bt.chunk_sample[x0 / kChunk];
@@ -1260,9 +1324,20 @@ int solver_args(const lbm3d_params *p, int dtype, const void *g, const int32_t *
   return LBM3D_OK;
 }
 
+// seed of step t's stochastic rounding (0: round to nearest)
+uint32_t sr_seed(const lbm3d_params &p, int64_t t) {
+  if (!(p.flags & LBM3D_FLAG_STOCHASTIC_ROUNDING)) return 0u;
+  uint64_t h = (uint64_t)t * 0x9E3779B97F4A7C15ull + 0x632BE59BD9B4E019ull;
+  h ^= h >> 31;
+  h *= 0xD6E8FEB86659FD93ull;
+  h ^= h >> 32;
+  return (uint32_t)h | 1u;
+}
+
 template <typename T, bool FORCED>
-void launch_step_parity(hipStream_t s, const KP &kp, void *g, const int32_t *cells, const uint4 *rows,
+void launch_step_parity(hipStream_t s, KP kp, void *g, const int32_t *cells, const uint4 *rows,
                         const uint32_t *face_bits, void *face_u, int64_t t) {
+  kp.sr_seed = sr_seed(kp.p, t);
   using R = typename St<T>::R;
   const unsigned grid = grid_for(kp.p.n_cells);
   if (t & 1)
@@ -1371,12 +1446,14 @@ int launch_batch_step(hipStream_t s, const KP &kp, const Batch &bt, void *g, con
                       const uint32_t *face_bits, void *face_u, int64_t t, const lbm3d_launch *cfg) {
   const lbm3d_launch &c = cfg ? cfg[t & 1] : kDefaultLaunch[t & 1];
   const int first = t == 0;
+  KP kpt = kp;
+  kpt.sr_seed = sr_seed(kp.p, t);
   if (kp.p.force_on) {
-    if (t & 1) launch_batch_parity<T, true, true>(s, kp, bt, g, cells, rows, face_bits, face_u, first, c);
-    else launch_batch_parity<T, false, true>(s, kp, bt, g, cells, rows, face_bits, face_u, first, c);
+    if (t & 1) launch_batch_parity<T, true, true>(s, kpt, bt, g, cells, rows, face_bits, face_u, first, c);
+    else launch_batch_parity<T, false, true>(s, kpt, bt, g, cells, rows, face_bits, face_u, first, c);
   } else {
-    if (t & 1) launch_batch_parity<T, true, false>(s, kp, bt, g, cells, rows, face_bits, face_u, first, c);
-    else launch_batch_parity<T, false, false>(s, kp, bt, g, cells, rows, face_bits, face_u, first, c);
+    if (t & 1) launch_batch_parity<T, true, false>(s, kpt, bt, g, cells, rows, face_bits, face_u, first, c);
+    else launch_batch_parity<T, false, false>(s, kpt, bt, g, cells, rows, face_bits, face_u, first, c);
   }
   return check_launch("lbm3d_batch_step");
 }
@@ -1435,6 +1512,7 @@ KP make_kp_batch(const lbm3d_params &p) {
   fill_phys(k.pd, p);
   k.S = slot_stride((uint32_t)p.n_cells);
   memset(&k.geo, 0, sizeof(k.geo));  // every sample's geometry comes from its descriptor
+  k.sr_seed = 0u;
   return k;
 }
 

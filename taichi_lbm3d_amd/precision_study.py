@@ -37,7 +37,7 @@ import time
 if __package__ in (None, ""):  # run as a file
     sys.path.insert(0, os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
 
-BYTES_PER_FLUP = {"f64": 304, "f32": 152, "f16": 76, "bf16": 76}
+BYTES_PER_FLUP = {"f64": 304, "f32": 152, "f16": 76, "bf16": 76, "f16sr": 76, "bf16sr": 76}
 
 
 def log(msg):
@@ -113,7 +113,7 @@ def main():
     ap.add_argument("--domain", choices=sorted(DOMAINS), required=True)
     ap.add_argument("--n", type=int, default=None, help="cube edge (default 256 / 512)")
     ap.add_argument("--seed", type=int, default=0)
-    ap.add_argument("--dtypes", default="f32,f16,bf16")
+    ap.add_argument("--dtypes", default="f32,f16,bf16,f16sr,bf16sr")
     ap.add_argument("--steps", type=int, default=None,
                     help="max steps to steady (pack256, default 50000) or fixed steps (channel512, default 4000)")
     ap.add_argument("--timed-steps", type=int, default=400)

@@ -28,7 +28,7 @@ def main():
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=4)
-    ap.add_argument("--dtype", default="f32", choices=["f32", "f16", "bf16"])
+    ap.add_argument("--dtype", default="f32", choices=["f32", "f16", "bf16", "f16sr", "bf16sr"])
     ap.add_argument("--calib-copy-mib", type=int, default=0)
     args = ap.parse_args()
 

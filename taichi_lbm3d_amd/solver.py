@@ -116,8 +116,10 @@ class LB3D_Solver_Single_Phase:
       convention -- "reference" (SP's tau = niu/3+1/2 and /3, /9 force term;
                     the default), "documented" or "textbook";
       dtype      -- population storage "f32" (default), "f16" or "bf16",
-                    stored shifted as f - w with FP32 arithmetic; or "f64",
-                    FP64 storage and arithmetic (the FP64 GPU reference run);
+                    stored shifted as f - w with FP32 arithmetic ("f16sr" /
+                    "bf16sr": the same with stochastic rounding of every
+                    store); or "f64", FP64 storage and arithmetic (the FP64
+                    GPU reference run);
       device     -- torch device index or string (default: current device).
     """
 
@@ -238,6 +240,7 @@ class LB3D_Solver_Single_Phase:
         p.guo_a, p.guo_b = a, b
         self.force_flag = 1 if (abs(self.fx) > 0 or abs(self.fy) > 0 or abs(self.fz) > 0) else 0
         p.force_on = self.force_flag
+        p.flags = _lib.STORAGE_FLAGS.get(self.dtype, 0)
         self.bc_vel_x_left = [self.vx_bcxl, self.vy_bcxl, self.vz_bcxl]
         self.bc_vel_x_right = [self.vx_bcxr, self.vy_bcxr, self.vz_bcxr]
         self.bc_vel_y_left = [self.vx_bcyl, self.vy_bcyl, self.vz_bcyl]
@@ -332,7 +335,7 @@ class LB3D_Solver_Single_Phase:
         torch = _torch()
         dev = self._torch_device()
         p = self._params
-        tdt = {"f32": torch.float32, "f16": torch.float16, "bf16": torch.bfloat16, "f64": torch.float64}[self.dtype]
+        tdt = _lib.torch_storage_dtype(self.dtype)
         stride = _lib.lib().size("lbm3d_slot_stride", p.n_cells)
         self._dev["g"] = torch.zeros(max(19 * stride, 1), dtype=tdt, device=dev)
         self._dev["face_u"] = torch.zeros(max(self._face_u_floats, 1), dtype=self._real_dtype(), device=dev)
