@@ -25,6 +25,9 @@
 #include "lbm3d.h"
 #include "lbm3d_conv.h"
 
+#ifndef LBM3D_CONV_TILED
+#define LBM3D_CONV_TILED 1  // 3x3x3 forward convolutions stage the input brick in LDS
+#endif
 #ifndef LBM3D_CONV_MW
 #define LBM3D_CONV_MW 2  // 16-voxel fragments per wave in the forward kernel
 #endif
@@ -218,6 +221,105 @@ __global__ void __launch_bounds__(256)
   }
 }
 
+// ---------------------------------------------------------------------------
+// Forward 3x3x3 convolution with the input tile in LDS.  A workgroup owns a
+// brick of 2 x 8 x 16 output voxels (z, y, x) of one volume and 16*NT output
+// channels.  For each chunk of 16 input channels it stages the brick's halo
+// (4 x 10 x 18 voxels) in LDS once; every tap then reads its A operands from
+// there instead of re-gathering them from memory.  One MFMA K step covers two
+// taps x 16 channels: lane group g takes tap 2s + g/2, channels 8*(g%2) .. +7.
+// Each wave computes 64 of the brick's voxels (4 fragments) x 16*NT channels.
+// ---------------------------------------------------------------------------
+constexpr int kBz = 2, kBy = 8, kBx = 16;                  // brick
+constexpr int kHz = kBz + 2, kHy = kBy + 2, kHx = kBx + 2;  // halo
+constexpr int kHalo = kHz * kHy * kHx;                     // 720 voxels
+
+template <int NT>
+__global__ void __launch_bounds__(256)
+    k_conv_fwd_tiled(const uint16_t *__restrict__ x, const uint16_t *__restrict__ w,
+                     const float *__restrict__ bias, uint16_t *__restrict__ y, const Vol vol, int cin, int cout,
+                     int kpad, int relu, int nbz, int nby, int nbx) {
+  __shared__ u32x4 halo[kHalo * 2];  // [voxel][16 channels] as two 16-byte halves
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int g = lane >> 4, r16 = lane & 15;
+  // brick coordinates
+  uint32_t bi = blockIdx.x;
+  const int bx = (int)(bi % (uint32_t)nbx);
+  bi /= (uint32_t)nbx;
+  const int by = (int)(bi % (uint32_t)nby);
+  bi /= (uint32_t)nby;
+  const int bz = (int)(bi % (uint32_t)nbz);
+  const int bb = (int)(bi / (uint32_t)nbz);
+  const int z0 = bz * kBz, y0 = by * kBy, x0 = bx * kBx;
+  const int co0 = blockIdx.y * 16 * NT;
+  // this lane's output voxels: brick-local index 64*wave + 16*f + r16
+  int lz[4], ly[4], lx[4];
+#pragma unroll
+  for (int f = 0; f < 4; ++f) {
+    const int lv = 64 * wave + 16 * f + r16;
+    lz[f] = lv / (kBy * kBx);
+    ly[f] = (lv / kBx) % kBy;
+    lx[f] = lv % kBx;
+  }
+  f32x4 acc[4][NT];
+#pragma unroll
+  for (int f = 0; f < 4; ++f)
+#pragma unroll
+    for (int t = 0; t < NT; ++t) acc[f][t] = f32x4{0.f, 0.f, 0.f, 0.f};
+  const int half = g & 1;
+  for (int c0 = 0; c0 < cin; c0 += 16) {
+    __syncthreads();
+    for (int e = threadIdx.x; e < kHalo * 2; e += 256) {
+      const int hv = e >> 1, hh = e & 1;
+      const int hz = hv / (kHy * kHx), hy = (hv / kHx) % kHy, hx = hv % kHx;
+      const int zz = z0 + hz - 1, yy = y0 + hy - 1, xx = x0 + hx - 1;
+      u32x4 val = u32x4{0u, 0u, 0u, 0u};
+      if (zz >= 0 && zz < vol.d && yy >= 0 && yy < vol.h && xx >= 0 && xx < vol.w)
+        val = ld16(x + ((((int64_t)bb * vol.d + zz) * vol.h + yy) * vol.w + xx) * cin + c0 + 8 * hh);
+      halo[e] = val;
+    }
+    __syncthreads();
+#pragma unroll 2
+    for (int tp = 0; tp < 28; tp += 2) {
+      const bool live = tp + (g >> 1) < 27;
+      const int tap = live ? tp + (g >> 1) : 26;  // the 28th slot of the last step is zero padding
+      const int dz = tap / 9 - 1, dy = (tap / 3) % 3 - 1, dx = tap % 3 - 1;
+      u32x4 a[4];
+#pragma unroll
+      for (int f = 0; f < 4; ++f) {
+        const int hv = ((lz[f] + dz + 1) * kHy + (ly[f] + dy + 1)) * kHx + (lx[f] + dx + 1);
+        a[f] = live ? halo[2 * hv + half] : u32x4{0u, 0u, 0u, 0u};
+      }
+#pragma unroll
+      for (int t = 0; t < NT; ++t) {
+        const uint16_t *wr = w + (size_t)(co0 + 16 * t + r16) * kpad + tap * cin + c0 + 8 * half;
+        const bf16x8 bv = __builtin_bit_cast(bf16x8, live ? ld16(wr) : u32x4{0u, 0u, 0u, 0u});
+#pragma unroll
+        for (int f = 0; f < 4; ++f)
+          acc[f][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, a[f]), bv, acc[f][t], 0, 0,
+                                                               0);
+      }
+    }
+  }
+  // C layout: lane holds rows (voxels) 4g .. 4g+3 of each fragment, column (channel) r16
+#pragma unroll
+  for (int t = 0; t < NT; ++t) {
+    const int co = co0 + 16 * t + r16;
+    const float bsum = bias ? bias[co] : 0.f;
+#pragma unroll
+    for (int f = 0; f < 4; ++f)
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        const int lv = 64 * wave + 16 * f + 4 * g + e;
+        const int oz = z0 + lv / (kBy * kBx), oy = y0 + (lv / kBx) % kBy, ox = x0 + lv % kBx;
+        if (oz >= vol.d || oy >= vol.h || ox >= vol.w) continue;
+        float val = acc[f][t][e] + bsum;
+        if (relu) val = val > 0.f ? val : 0.f;
+        y[((((size_t)bb * vol.d + oz) * vol.h + oy) * vol.w + ox) * cout + co] = to_bf16(val);
+      }
+  }
+}
+
 // wt[ci][tap'*cout + co] = w[co][tap*cin + ci], tap' = taps-1-tap; zero padding columns
 __global__ void k_wflip(const uint16_t *__restrict__ w, uint16_t *__restrict__ wt, int cin, int cout, int taps,
                         int kpad_in, int kpad_out) {
@@ -376,6 +478,21 @@ int lbm3d_conv3d_fwd(void *stream, const uint16_t *x, const uint16_t *w, const f
   const Vol vol = make_vol(n, d, h, wd);
   const int kpad = lbm3d_conv_kpad(cin, ks);
   hipStream_t s = (hipStream_t)stream;
+  if (ks == 3 && LBM3D_CONV_TILED) {
+    const int nbz = (d + kBz - 1) / kBz, nby = (h + kBy - 1) / kBy, nbx = (wd + kBx - 1) / kBx;
+    const int64_t bricks = (int64_t)n * nbz * nby * nbx;
+    if (bricks > (int64_t)(UINT32_MAX / 256u)) return cfail(LBM3D_ESIZE, "too many bricks for one launch");
+    if (cout % 64 == 0)
+      k_conv_fwd_tiled<4><<<dim3((unsigned)bricks, cout / 64), 256, 0, s>>>(x, w, bias, y, vol, cin, cout, kpad, relu,
+                                                                             nbz, nby, nbx);
+    else if (cout % 32 == 0)
+      k_conv_fwd_tiled<2><<<dim3((unsigned)bricks, cout / 32), 256, 0, s>>>(x, w, bias, y, vol, cin, cout, kpad, relu,
+                                                                             nbz, nby, nbx);
+    else
+      k_conv_fwd_tiled<1><<<dim3((unsigned)bricks, cout / 16), 256, 0, s>>>(x, w, bias, y, vol, cin, cout, kpad, relu,
+                                                                             nbz, nby, nbx);
+    return ccheck("lbm3d_conv3d_fwd");
+  }
   constexpr int MW = LBM3D_CONV_MW;
   const unsigned gx = (vol.vox + 64u * MW - 1u) / (64u * MW);
   if (cout % 64 == 0)

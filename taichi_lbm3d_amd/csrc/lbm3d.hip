@@ -290,14 +290,15 @@ struct TableRows {
   }
 };
 // From a workgroup's LDS copy of the rows i0-1 .. i0+1 x j0-1 .. j0+J+1 (JW = J + 3
-// rows per plane), for a cell of row (i0, j0 + jrel):
+// rows per plane, W words per row), for cell k of row (i0, j0 + jrel): c points
+// at the cell's own word, and the other 8 sit a wave-uniform stride away
+// (si = JW W words per plane, sj = W per row).
 struct WindowRows {
-  const uint4 *win;
-  uint32_t W, JW;
-  int jrel;
-  __device__ __forceinline__ uint4 word(int di, int dj, int k) const {
-    return win[((uint32_t)(di + 1) * JW + (uint32_t)(jrel + dj + 1)) * W + ((uint32_t)k >> 5)];
-  }
+  const uint4 *c;
+  int si, sj;
+  __device__ __forceinline__ WindowRows(const uint4 *win, uint32_t W, uint32_t JW, int jrel, int k)
+      : c(win + (((JW + (uint32_t)jrel + 1u) * W) + ((uint32_t)k >> 5))), si((int)(JW * W)), sj((int)W) {}
+  __device__ __forceinline__ uint4 word(int di, int dj, int) const { return c[di * si + dj * sj]; }
 };
 
 // Compact ids of the 18 neighbours (index q-1), -1 where the neighbour is
@@ -408,8 +409,6 @@ template <> struct St<__hip_bfloat16> {
                                                uint32_t rnd) {
     __builtin_amdgcn_raw_buffer_store_b16(bf16_sr(v, rnd), r, (int)vo, (int)so, LBM3D_CPOL_STORE);
   }
-    __builtin_amdgcn_raw_buffer_store_b16(b, r, (int)vo, (int)so, LBM3D_CPOL_STORE);
-  }
   static __device__ __forceinline__ float ld(__amdgpu_buffer_rsrc_t r, uint32_t vo, uint32_t so) {
     const unsigned short b = __builtin_amdgcn_raw_buffer_load_b16(r, (int)vo, (int)so, LBM3D_CPOL_LOAD);
     return __uint_as_float((uint32_t)b << 16);
@@ -436,7 +435,6 @@ template <> struct St<double> {
   }
 };
 
-// Resource k covers slots 2k-1 and 2k (k = 1..9); resource 0 covers slot 0.
 // 32-bit mixing function (a "lowbias32" integer hash)
 __device__ __forceinline__ uint32_t mix32(uint32_t h) {
   h ^= h >> 16;
@@ -447,6 +445,7 @@ __device__ __forceinline__ uint32_t mix32(uint32_t h) {
   return h;
 }
 
+// Resource k covers slots 2k-1 and 2k (k = 1..9); resource 0 covers slot 0.
 template <typename T>
 struct Slots {
   using R = typename St<T>::R;
@@ -684,7 +683,8 @@ struct RowView {
   const Rows *rb;
   const uint4 *win;
   uint32_t JW;
-  int j0;
+  int i0, j0;
+  uint32_t c0;  // dense index of cell (i0, j0, 0)
 };
 
 template <typename T, bool ODD>
@@ -697,13 +697,19 @@ __device__ __forceinline__ void read_cell(const Slots<T> &sl, const Geo &gm, con
 #pragma unroll
     for (int q = 0; q < kQ; ++q) v[q] = sl.ld_slot(q, xb);
   } else {
-    coords(gm, (uint32_t)cells[x], cc.i, cc.j, cc.k);
     cc.have_coords = true;
     int32_t nb[18];
     if (rv.win) {
-      const WindowRows src{rv.win, gm.W, rv.JW, cc.j - rv.j0};
+      // the window's cells all lie in plane i0, rows j0 .. j0+J
+      const uint32_t d = (uint32_t)cells[x] - rv.c0;
+      const uint32_t jrel = fdiv(d, gm.m_nz, gm.s_nz);
+      cc.i = rv.i0;
+      cc.j = rv.j0 + (int)jrel;
+      cc.k = (int)(d - jrel * (uint32_t)gm.nz);
+      const WindowRows src(rv.win, gm.W, rv.JW, (int)jrel, cc.k);
       neighbor_ids(src, gm, cc.k, nb);
     } else {
+      coords(gm, (uint32_t)cells[x], cc.i, cc.j, cc.k);
       const TableRows src(*rv.rb, gm, cc.i, cc.j);
       neighbor_ids(src, gm, cc.k, nb);
     }
@@ -864,9 +870,9 @@ __device__ __forceinline__ RowView row_window(const Geo &gm, const int32_t *__re
       win[e] = rows[row * gm.W + w];
     }
     __syncthreads();
-    return RowView{&rb, win, JW, j0};
+    return RowView{&rb, win, JW, i0, j0, ((uint32_t)i0 * (uint32_t)gm.ny + (uint32_t)j0) * (uint32_t)gm.nz};
   }
-  return RowView{&rb, nullptr, 0u, 0};
+  return RowView{&rb, nullptr, 0u, 0, 0, 0u};
 }
 
 template <typename T, bool ODD, bool FORCED>
@@ -880,7 +886,7 @@ __global__ void __launch_bounds__(kBlock, LBM3D_STEP_MINB)
   const uint32_t x = x0 + threadIdx.x;
   const Slots<T> sl(g, kp.S, kp.sr_seed);
   const Rows rb(rows, kp.geo.n_row_words);
-  RowView rv{&rb, nullptr, 0u, 0};
+  RowView rv{&rb, nullptr, 0u, 0, 0, 0u};
   if constexpr (ODD && LBM3D_ROW_WINDOW) {
     __shared__ uint4 win[kWinWords];
     rv = row_window(kp.geo, cells, rows, rb, x0, n, win);
@@ -900,7 +906,7 @@ __global__ void __launch_bounds__(kBlock)
   if (x >= N) return;
   const Slots<T> sl(g, kp.S);
   const Rows rb(rows, kp.geo.n_row_words);
-  const RowView rv{&rb, nullptr, 0u, 0};
+  const RowView rv{&rb, nullptr, 0u, 0, 0, 0u};
   macros_cell<T, ODD>(kp, kp.geo, sl, rv, cells, face_bits, face_u, x, first, rho_out, v_out, N);
 }
 
@@ -955,7 +961,7 @@ __global__ void __launch_bounds__(B)
     const uint32_t x = x0 + threadIdx.x;
     if (x >= d.cell_end) continue;
     const Rows rb(rows + d.rows_begin, d.geo.n_row_words);
-    const RowView rv{&rb, nullptr, 0u, 0};
+    const RowView rv{&rb, nullptr, 0u, 0, 0, 0u};
     step_cell<T, ODD, FORCED>(kp, d.geo, sl, rv, cells, face_bits, face_u, x, first);
   }
 }
@@ -972,7 +978,7 @@ __global__ void __launch_bounds__(kBlock)
   if (x >= d.cell_end) return;
   const Slots<T> sl(g, kp.S);
   const Rows rb(rows + d.rows_begin, d.geo.n_row_words);
-  const RowView rv{&rb, nullptr, 0u, 0};
+  const RowView rv{&rb, nullptr, 0u, 0, 0, 0u};
   macros_cell<T, ODD>(kp, d.geo, sl, rv, cells, face_bits, face_u, x, first, rho_out, v_out, (uint32_t)kp.p.n_cells);
 }
 
