@@ -16,7 +16,8 @@
  *                to kpad = lbm3d_conv_kpad(cin, ks) (a multiple of 32);
  *   bias         FP32 b[cout] (may be NULL);
  *   weight grads FP32 dw[co][kpad].
- * Convolutions are "same" (stride 1, zero padding ks/2).  cin and cout must
+ * Convolutions are "same" (stride 1, zero padding ks/2, or periodic wrap for
+ * the forward convolution with LBM3D_CONV_PERIODIC).  cin and cout must
  * be multiples of 16, at most 256; a batch holds fewer than 2^31 voxels.
  * Return 0 or a negative LBM3D_E* code (lbm3d.h); lbm3d_conv_last_error()
  * gives the message.
@@ -36,10 +37,16 @@ const char *lbm3d_conv_last_error(void);
 /* Padded row length of the weight matrix: ks^3 * cin rounded up to 32. */
 int32_t lbm3d_conv_kpad(int32_t cin, int32_t ks);
 
-/* y = act(conv(x, w) + bias), act = ReLU when relu != 0, BF16 out (FP32 accumulation). */
+/* Flags of lbm3d_conv3d_fwd. */
+#define LBM3D_CONV_RELU 1     /* apply ReLU to the output */
+#define LBM3D_CONV_PERIODIC 2 /* wrap taps past a face to the opposite face instead of zero padding */
+
+/* y = act(conv(x, w) + bias), act = ReLU with LBM3D_CONV_RELU, BF16 out (FP32
+ * accumulation); with LBM3D_CONV_PERIODIC each volume is treated as one period
+ * of a periodic field (no padding needed around periodic geometries). */
 int lbm3d_conv3d_fwd(void *stream, const uint16_t *x, const uint16_t *w, const float *bias, uint16_t *y,
                      int32_t n, int32_t d, int32_t h, int32_t wd, int32_t cin, int32_t cout, int32_t ks,
-                     int32_t relu);
+                     int32_t flags);
 
 /* Weights of the data-gradient convolution: wt[ci][tap'*cout + co] =
  * w[co][tap*cin + ci] with tap' = ks^3 - 1 - tap (the kernel flipped and

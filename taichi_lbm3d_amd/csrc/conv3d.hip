@@ -72,11 +72,13 @@ __device__ __forceinline__ uint32_t qdiv(uint32_t c, const Div &v) { return (uin
 struct Vol {
   int32_t n, d, h, w;
   Div dw, dh, dd;
-  uint32_t vox;  // n*d*h*w
+  uint32_t vox;      // n*d*h*w
+  int32_t periodic;  // 1: taps past a face wrap to the opposite face; 0: zero padding
 };
 
-Vol make_vol(int32_t n, int32_t d, int32_t h, int32_t w) {
+Vol make_vol(int32_t n, int32_t d, int32_t h, int32_t w, int32_t periodic = 0) {
   Vol v;
+  v.periodic = periodic;
   v.n = n;
   v.d = d;
   v.h = h;
@@ -110,10 +112,20 @@ __device__ __forceinline__ void tap_offset(int tap, int ks, int &dz, int &dy, in
   }
 }
 
-// neighbour voxel of (b,z,y,x) at tap offset, or -1 outside the volume (zero padding)
+// a coordinate one step outside [0, n) wrapped back in
+__device__ __forceinline__ int wrap1(int a, int n) { return a < 0 ? a + n : (a >= n ? a - n : a); }
+
+// neighbour voxel of (b,z,y,x) at tap offset (|offset| <= 1), wrapped when the
+// volume is periodic, else -1 outside the volume (zero padding)
 __device__ __forceinline__ int64_t shifted(const Vol &v, int b, int z, int y, int x, int dz, int dy, int dx) {
-  const int z2 = z + dz, y2 = y + dy, x2 = x + dx;
-  if (z2 < 0 || z2 >= v.d || y2 < 0 || y2 >= v.h || x2 < 0 || x2 >= v.w) return -1;
+  int z2 = z + dz, y2 = y + dy, x2 = x + dx;
+  if (v.periodic) {
+    z2 = wrap1(z2, v.d);
+    y2 = wrap1(y2, v.h);
+    x2 = wrap1(x2, v.w);
+  } else if (z2 < 0 || z2 >= v.d || y2 < 0 || y2 >= v.h || x2 < 0 || x2 >= v.w) {
+    return -1;
+  }
   return (((int64_t)b * v.d + z2) * v.h + y2) * v.w + x2;
 }
 
@@ -272,7 +284,12 @@ __global__ void __launch_bounds__(256)
     for (int e = threadIdx.x; e < kHalo * 2; e += 256) {
       const int hv = e >> 1, hh = e & 1;
       const int hz = hv / (kHy * kHx), hy = (hv / kHx) % kHy, hx = hv % kHx;
-      const int zz = z0 + hz - 1, yy = y0 + hy - 1, xx = x0 + hx - 1;
+      int zz = z0 + hz - 1, yy = y0 + hy - 1, xx = x0 + hx - 1;
+      if (vol.periodic) {  // halo voxels past a face come from the opposite face
+        zz = wrap1(zz, vol.d);
+        yy = wrap1(yy, vol.h);
+        xx = wrap1(xx, vol.w);
+      }
       u32x4 val = u32x4{0u, 0u, 0u, 0u};
       if (zz >= 0 && zz < vol.d && yy >= 0 && yy < vol.h && xx >= 0 && xx < vol.w)
         val = ld16(x + ((((int64_t)bb * vol.d + zz) * vol.h + yy) * vol.w + xx) * cin + c0 + 8 * hh);
@@ -471,11 +488,13 @@ int32_t lbm3d_conv_kpad(int32_t cin, int32_t ks) {
 }
 
 int lbm3d_conv3d_fwd(void *stream, const uint16_t *x, const uint16_t *w, const float *bias, uint16_t *y, int32_t n,
-                     int32_t d, int32_t h, int32_t wd, int32_t cin, int32_t cout, int32_t ks, int32_t relu) {
+                     int32_t d, int32_t h, int32_t wd, int32_t cin, int32_t cout, int32_t ks, int32_t flags) {
   int rc = check_conv(n, d, h, wd, cin, cout, ks);
   if (rc) return rc;
   if (!x || !w || !y) return cfail(LBM3D_EINVAL, "NULL tensor");
-  const Vol vol = make_vol(n, d, h, wd);
+  if (flags & ~(LBM3D_CONV_RELU | LBM3D_CONV_PERIODIC)) return cfail(LBM3D_EINVAL, "unknown flag bits");
+  const int relu = (flags & LBM3D_CONV_RELU) ? 1 : 0;
+  const Vol vol = make_vol(n, d, h, wd, (flags & LBM3D_CONV_PERIODIC) ? 1 : 0);
   const int kpad = lbm3d_conv_kpad(cin, ks);
   hipStream_t s = (hipStream_t)stream;
   if (ks == 3 && LBM3D_CONV_TILED) {

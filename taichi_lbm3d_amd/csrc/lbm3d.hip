@@ -382,8 +382,14 @@ __device__ __forceinline__ unsigned short half_sr(float v, uint32_t rnd) {
     const unsigned short sign = (unsigned short)((bits >> 16) & 0x8000u);
     return (float)(rnd >> 8) * 0x1p-24f < p ? (unsigned short)(sign | 1u) : sign;
   }
-  const uint32_t d = bits + (rnd & ((1u << drop) - 1u));
-  return __half_as_ushort(__float2half_rz(__uint_as_float(d)));
+  // dither below the last place, then clear those bits: the result is a half
+  // value (exact under the conversion), one grid step up with probability equal
+  // to the dropped fraction.  The bits are cleared rather than left to a
+  // toward-zero conversion because a carry into the next binade of the
+  // subnormal range keeps the same 2^-24 grid there.
+  const uint32_t mask = (1u << drop) - 1u;
+  const uint32_t d = (bits + (rnd & mask)) & ~mask;
+  return __half_as_ushort(__float2half_rn(__uint_as_float(d)));
 }
 
 template <> struct St<__half> {

@@ -49,13 +49,19 @@ def _stream(dev):
 
 # ---- the convolution as an autograd function ----------------------------------------------
 
-def conv_fwd(x, w_bf16, bias, cout, ks, relu):
-    """x (N,X,Y,Z,Cin) bf16 contiguous, w_bf16 (cout, kpad) bf16 -> (N,X,Y,Z,cout) bf16."""
+CONV_RELU, CONV_PERIODIC = 1, 2  # lbm3d_conv3d_fwd flags
+
+
+def conv_fwd(x, w_bf16, bias, cout, ks, relu, periodic=False):
+    """x (N,X,Y,Z,Cin) bf16 contiguous, w_bf16 (cout, kpad) bf16 -> (N,X,Y,Z,cout) bf16.
+    periodic: each volume is one period of a periodic field (taps wrap instead of
+    reading zero padding)."""
     torch = _torch()
     n, d, h, wd, cin = x.shape
     y = torch.empty((n, d, h, wd, cout), dtype=torch.bfloat16, device=x.device)
+    flags = (CONV_RELU if relu else 0) | (CONV_PERIODIC if periodic else 0)
     _lib.lib().call("lbm3d_conv3d_fwd", _stream(x.device), _vp(x), _vp(w_bf16),
-                    _vp(bias) if bias is not None else None, _vp(y), n, d, h, wd, cin, cout, ks, int(relu))
+                    _vp(bias) if bias is not None else None, _vp(y), n, d, h, wd, cin, cout, ks, flags)
     return y
 
 
@@ -93,16 +99,18 @@ def _make_conv_function():
         """y = act(conv3d(x, w) + b) with the MFMA kernels in both directions."""
 
         @staticmethod
-        def forward(ctx, x, w, b, ks, relu, need_dx):
+        def forward(ctx, x, w, b, ks, relu, need_dx, periodic):
             x = x.contiguous()
             wb = w.to(torch.bfloat16).contiguous()
-            y = conv_fwd(x, wb, b, w.shape[0], ks, relu)
+            y = conv_fwd(x, wb, b, w.shape[0], ks, relu, periodic)
             ctx.save_for_backward(x, wb, y if relu else None)
-            ctx.ks, ctx.relu, ctx.need_dx = ks, relu, need_dx
+            ctx.ks, ctx.relu, ctx.need_dx, ctx.periodic = ks, relu, need_dx, periodic
             return y
 
         @staticmethod
         def backward(ctx, gy):
+            if ctx.periodic:
+                raise NotImplementedError("periodic convolutions are forward-only (inference)")
             x, wb, y = ctx.saved_tensors
             g = gy.contiguous().to(torch.bfloat16)
             if ctx.relu:
@@ -110,7 +118,7 @@ def _make_conv_function():
             dw = conv_wgrad(x, g, ctx.ks)
             db = g.float().sum(dim=(0, 1, 2, 3))
             dx = conv_dgrad(g, wb, x.shape[-1], ctx.ks) if ctx.need_dx else None
-            return dx, dw, db, None, None, None
+            return dx, dw, db, None, None, None, None
 
     return Conv3dMFMA
 
@@ -118,11 +126,11 @@ def _make_conv_function():
 _CONV = None
 
 
-def conv3d(x, w, b, ks: int = 3, relu: bool = True, need_dx: bool = True):
+def conv3d(x, w, b, ks: int = 3, relu: bool = True, need_dx: bool = True, periodic: bool = False):
     global _CONV
     if _CONV is None:
         _CONV = _make_conv_function()
-    return _CONV.apply(x, w, b, ks, relu, need_dx)
+    return _CONV.apply(x, w, b, ks, relu, need_dx, periodic)
 
 
 # ---- the network ------------------------------------------------------------------------------
@@ -181,22 +189,25 @@ class UNet3D:
     def parameters(self):
         return list(self.params.values())
 
-    def _c(self, name, x, relu=True, need_dx=True):
+    def _c(self, name, x, relu=True, need_dx=True, periodic=False):
         cin, cout, ks = self.specs[name]
-        return conv3d(x, self.params[name + ".w"], self.params[name + ".b"], ks, relu, need_dx)
+        return conv3d(x, self.params[name + ".w"], self.params[name + ".b"], ks, relu, need_dx, periodic)
 
-    def forward(self, x):
+    def forward(self, x, periodic: bool = False):
         """x (N,X,Y,Z,16) bf16 with channels 0-1 the inputs -> (N,X,Y,Z,4) float32.
-        X, Y, Z must be multiples of 8."""
+        X, Y, Z must be multiples of 8.  periodic: every volume is one period of a
+        periodic geometry, and the convolutions wrap around its faces instead of
+        zero padding (forward only)."""
         torch = _torch()
-        e1 = self._c("e1b", self._c("e1a", x, need_dx=False))
-        e2 = self._c("e2b", self._c("e2a", _pool(e1)))
-        e3 = self._c("e3b", self._c("e3a", _pool(e2)))
-        e4 = self._c("e4b", self._c("e4a", _pool(e3)))
-        d3 = self._c("d3b", self._c("d3a", torch.cat([_up(e4), e3], dim=-1)))
-        d2 = self._c("d2b", self._c("d2a", torch.cat([_up(d3), e2], dim=-1)))
-        d1 = self._c("d1b", self._c("d1a", torch.cat([_up(d2), e1], dim=-1)))
-        out = self._c("head", d1, relu=False)
+        p = periodic
+        e1 = self._c("e1b", self._c("e1a", x, need_dx=False, periodic=p), periodic=p)
+        e2 = self._c("e2b", self._c("e2a", _pool(e1), periodic=p), periodic=p)
+        e3 = self._c("e3b", self._c("e3a", _pool(e2), periodic=p), periodic=p)
+        e4 = self._c("e4b", self._c("e4a", _pool(e3), periodic=p), periodic=p)
+        d3 = self._c("d3b", self._c("d3a", torch.cat([_up(e4), e3], dim=-1), periodic=p), periodic=p)
+        d2 = self._c("d2b", self._c("d2a", torch.cat([_up(d3), e2], dim=-1), periodic=p), periodic=p)
+        d1 = self._c("d1b", self._c("d1a", torch.cat([_up(d2), e1], dim=-1), periodic=p), periodic=p)
+        out = self._c("head", d1, relu=False, periodic=p)
         return out[..., :4].float()
 
     __call__ = forward
@@ -291,23 +302,36 @@ def periodic_pad(t, pad):
     return t
 
 
-def predict(net, solids, fx, nu_eff, batch: int = 2):
+def predict(net, solids, fx, nu_eff, batch: int = 2, wrap=None):
     """Warm-start (rho = 1, v) device tensors predicted for periodic geometries of
-    one shape, `batch` volumes per forward pass.  Each volume is padded
-    periodically by at least BORDER cells (to a multiple of 8) so every cell is
-    predicted with its true surroundings; the centre is kept."""
+    one shape, `batch` volumes per forward pass.
+
+    wrap=True runs the network's convolutions periodically on the volume itself
+    (every extent a multiple of 8): every cell sees its true periodic
+    surroundings.  wrap=False pads each volume periodically by at least BORDER
+    cells (to a multiple of 8), runs the zero-padded network and keeps the
+    centre.  The default wraps when the extents allow it."""
     torch = _torch()
     out = []
+    if not solids:
+        return out
+    shape = np.asarray(solids[0]).shape
+    if wrap is None:
+        wrap = all(n % 8 == 0 for n in shape)
+    if wrap and any(n % 8 for n in shape):
+        raise ValueError(f"periodic inference needs extents that are multiples of 8, got {shape}")
     with torch.no_grad():
         for b0 in range(0, len(solids), batch):
             group = solids[b0:b0 + batch]
-            shape = np.asarray(group[0]).shape
-            pads = [BORDER + (-(n + 2 * BORDER)) % 8 // 2 for n in shape]
-            pad = max(pads)
-            while any((n + 2 * pad) % 8 for n in shape):
-                pad += 1
+            if wrap:
+                pad = 0
+            else:
+                pads = [BORDER + (-(n + 2 * BORDER)) % 8 // 2 for n in shape]
+                pad = max(pads)
+                while any((n + 2 * pad) % 8 for n in shape):
+                    pad += 1
             inp = torch.stack([periodic_pad(make_inputs(s, device=net.device), pad) for s in group])
-            pred = net(to_network_input(inp))
+            pred = net(to_network_input(inp), periodic=wrap)
             X, Y, Z = shape
             for i, s in enumerate(group):
                 fluid = torch.as_tensor(np.asarray(s) == 0, device=net.device)

@@ -72,6 +72,28 @@ def test_conv_forward_matches_cpu_fp32(gpu, shape, relu):
     assert rel < 4e-3, rel
 
 
+@pytest.mark.parametrize("shape", SHAPES + [(1, 1, 2, 3, 16, 16, 3), (1, 2, 9, 17, 16, 32, 3)])
+def test_conv_forward_periodic_matches_circular_cpu_fp32(gpu, shape):
+    """Periodic forward convolution against a CPU convolution of the circularly
+    padded input (one-voxel volumes wrap onto themselves)."""
+    import torch.nn.functional as F
+    from taichi_lbm3d_amd.surrogate import conv_fwd
+    n, d, h, w, cin, cout, ks = shape
+    x, wt, b = make_case(shape, 15)
+    y = conv_fwd(x.to(gpu), wt.to(gpu), b.to(gpu), cout, ks, True, periodic=True).float().cpu()
+    xt = x.float().permute(0, 4, 1, 2, 3)
+    p = ks // 2
+    if p:
+        # circular padding by one voxel, axis by axis (valid for extents of 1 too)
+        for dim in (2, 3, 4):
+            m = xt.shape[dim]
+            idx = _torch().tensor([m - 1] + list(range(m)) + [0])
+            xt = xt.index_select(dim, idx)
+    ref = F.conv3d(xt, torch_weight(wt, cin, cout, ks), b, padding=0).relu().permute(0, 2, 3, 4, 1)
+    err = (y - ref).abs().max().item()
+    assert err <= 2 ** -8 * ref.abs().max().item() + 1e-6, err
+
+
 @pytest.mark.parametrize("shape", SHAPES)
 def test_conv_data_gradient_matches_cpu_autograd(gpu, shape):
     torch = _torch()
@@ -162,3 +184,23 @@ def test_unet_gradients_match_cpu_fp32(gpu):
         cos = float(ga @ gb / (ga.norm() * gb.norm()))
         assert cos > 0.99, (k, cos)
         assert abs(ga.norm() / gb.norm() - 1) < 0.05, (k, float(ga.norm() / gb.norm()))
+
+
+def test_unet_periodic_inference_is_translation_equivariant(gpu):
+    """With periodic convolutions, rolling a periodic input by a multiple of 8
+    voxels rolls the prediction by the same amount, bit for bit."""
+    torch = _torch()
+    from taichi_lbm3d_amd import surrogate as sg
+    net = sg.UNet3D(gpu, seed=3)
+    g = torch.Generator().manual_seed(4)
+    x = torch.zeros((1, 32, 24, 16, sg.CIN_PAD), dtype=torch.bfloat16)
+    x[..., :2] = torch.rand((1, 32, 24, 16, 2), generator=g).to(torch.bfloat16)
+    x = x.to(gpu)
+    with torch.no_grad():
+        y = net(x, periodic=True)
+        yr = net(torch.roll(x, shifts=(8, 16, 8), dims=(1, 2, 3)).contiguous(), periodic=True)
+    assert torch.equal(torch.roll(y, shifts=(8, 16, 8), dims=(1, 2, 3)), yr)
+    # and zero padding breaks that symmetry at the faces
+    with torch.no_grad():
+        y0 = net(x)
+    assert not torch.equal(y0, y)
