@@ -37,11 +37,15 @@ constexpr int kBlock = LBM3D_BLOCK;  // threads per block, a multiple of 64
 constexpr int kWinWords = 384;       // LDS row window: 3 planes x (J+3) rows x W words
 
 // Build-time knobs (defaults are the measured best on MI355X).
-#ifndef LBM3D_XCD_REMAP_EVEN
-#define LBM3D_XCD_REMAP_EVEN 0  // contiguous block range per XCD on even steps
+#ifndef LBM3D_XCD_CHUNK_EVEN
+#define LBM3D_XCD_CHUNK_EVEN 0  // C > 0: each XCD takes runs of C consecutive blocks on even steps
 #endif
-#ifndef LBM3D_XCD_REMAP_ODD
-#define LBM3D_XCD_REMAP_ODD 0   // ... and on odd steps
+#ifndef LBM3D_XCD_CHUNK_ODD
+// ... and on odd steps (0: the hardware's round robin).  An odd step's cells
+// read and write slot lines shared with the cells one plane away (and the
+// bounce-back slots of their own lines); 256-block runs keep most of those
+// sharers on one XCD's L2: 239 instead of 251 us per odd step on a 256^3 pack.
+#define LBM3D_XCD_CHUNK_ODD 256
 #endif
 #ifndef LBM3D_STEP_MINB
 #define LBM3D_STEP_MINB 1
@@ -208,25 +212,27 @@ __device__ __forceinline__ void coords(const Geo &gm, uint32_t c, int &i, int &j
 // periodic_index, SP:247-257: -1 -> n-1 and n -> 0 on every axis, whatever the BC mode
 __device__ __forceinline__ int wrap(int a, int n) { return a < 0 ? n - 1 : (a > n - 1 ? 0 : a); }
 
-// Block -> cell-range map.  Workgroups are dealt to the 8 XCDs round robin;
-// with the grid padded to a multiple of 8 the remap hands XCD k the k-th
-// contiguous eighth of the block range, so blocks sharing cache lines of a
-// slot share an L2.
-template <bool REMAP, bool REVERSE = false>
+// Block -> cell-range map.  Workgroups are dealt to the 8 XCDs round robin
+// (hardware block h runs on XCD h % 8).  With C > 0 the map hands each XCD runs
+// of C consecutive blocks: within every group of 8C hardware blocks, XCD k
+// takes logical blocks kC .. kC+C-1, so blocks sharing cache lines of a slot
+// mostly share an L2 while all XCDs still sweep the range together.  The grid
+// is padded to a multiple of 8C blocks.
+template <uint32_t C, bool REVERSE = false>
 __device__ __forceinline__ uint32_t logical_block() {
   const uint32_t h = blockIdx.x;
-  if constexpr (REMAP) {
-    const uint32_t per = gridDim.x >> 3;
-    const uint32_t r = REVERSE ? per - 1u - (h >> 3) : (h >> 3);
-    return (h & 7u) * per + r;
-  } else {
-    return REVERSE ? gridDim.x - 1u - h : h;
+  uint32_t b = h;
+  if constexpr (C > 0) {
+    const uint32_t r = h % (8u * C);
+    b = (h - r) + (r & 7u) * C + (r >> 3);
   }
+  return REVERSE ? gridDim.x - 1u - b : b;
 }
 
-inline unsigned grid_for(int64_t n) {
+inline unsigned grid_for(int64_t n, int64_t chunk = 1) {
   const int64_t b = (n + kBlock - 1) / kBlock;
-  return (unsigned)((b + 7) & ~int64_t(7));
+  const int64_t per = 8 * (chunk > 0 ? chunk : 1);
+  return (unsigned)((b + per - 1) / per * per);
 }
 
 // ---------------------------------------------------------------------------
@@ -691,6 +697,7 @@ struct RowView {
   uint32_t JW;
   int i0, j0;
   uint32_t c0;  // dense index of cell (i0, j0, 0)
+  int32_t cell; // >= 0: the cell's dense index, loaded ahead (else read from cells[])
 };
 
 template <typename T, bool ODD>
@@ -705,9 +712,10 @@ __device__ __forceinline__ void read_cell(const Slots<T> &sl, const Geo &gm, con
   } else {
     cc.have_coords = true;
     int32_t nb[18];
+    const uint32_t dense = rv.cell >= 0 ? (uint32_t)rv.cell : (uint32_t)cells[x];
     if (rv.win) {
       // the window's cells all lie in plane i0, rows j0 .. j0+J
-      const uint32_t d = (uint32_t)cells[x] - rv.c0;
+      const uint32_t d = dense - rv.c0;
       const uint32_t jrel = fdiv(d, gm.m_nz, gm.s_nz);
       cc.i = rv.i0;
       cc.j = rv.j0 + (int)jrel;
@@ -715,7 +723,7 @@ __device__ __forceinline__ void read_cell(const Slots<T> &sl, const Geo &gm, con
       const WindowRows src(rv.win, gm.W, rv.JW, (int)jrel, cc.k);
       neighbor_ids(src, gm, cc.k, nb);
     } else {
-      coords(gm, (uint32_t)cells[x], cc.i, cc.j, cc.k);
+      coords(gm, dense, cc.i, cc.j, cc.k);
       const TableRows src(*rv.rb, gm, cc.i, cc.j);
       neighbor_ids(src, gm, cc.k, nb);
     }
@@ -876,26 +884,30 @@ __device__ __forceinline__ RowView row_window(const Geo &gm, const int32_t *__re
       win[e] = rows[row * gm.W + w];
     }
     __syncthreads();
-    return RowView{&rb, win, JW, i0, j0, ((uint32_t)i0 * (uint32_t)gm.ny + (uint32_t)j0) * (uint32_t)gm.nz};
+    return RowView{&rb, win, JW, i0, j0, ((uint32_t)i0 * (uint32_t)gm.ny + (uint32_t)j0) * (uint32_t)gm.nz, -1};
   }
-  return RowView{&rb, nullptr, 0u, 0, 0, 0u};
+  return RowView{&rb, nullptr, 0u, 0, 0, 0u, -1};
 }
 
 template <typename T, bool ODD, bool FORCED>
 __global__ void __launch_bounds__(kBlock, LBM3D_STEP_MINB)
     k_step(const KP kp, T *__restrict__ g, const int32_t *__restrict__ cells, const uint4 *__restrict__ rows,
            const uint32_t *__restrict__ face_bits, typename St<T>::R *__restrict__ face_u, int first) {
-  const uint32_t x0 = logical_block<ODD ? LBM3D_XCD_REMAP_ODD : LBM3D_XCD_REMAP_EVEN,
+  const uint32_t x0 = logical_block<ODD ? LBM3D_XCD_CHUNK_ODD : LBM3D_XCD_CHUNK_EVEN,
                                     !ODD && LBM3D_REVERSE_EVEN>() * kBlock;
   const uint32_t n = (uint32_t)kp.p.n_cells;
   if (x0 >= n) return;  // whole padding blocks of the grid
   const uint32_t x = x0 + threadIdx.x;
   const Slots<T> sl(g, kp.S, kp.sr_seed);
   const Rows rb(rows, kp.geo.n_row_words);
-  RowView rv{&rb, nullptr, 0u, 0, 0, 0u};
+  RowView rv{&rb, nullptr, 0u, 0, 0, 0u, -1};
   if constexpr (ODD && LBM3D_ROW_WINDOW) {
+    // the cell's dense index is loaded before the window is staged, so the two
+    // loads overlap instead of following each other
+    const int32_t own = cells[x < n ? x : n - 1];
     __shared__ uint4 win[kWinWords];
     rv = row_window(kp.geo, cells, rows, rb, x0, n, win);
+    rv.cell = own;
   }
   if (x >= n) return;
   step_cell<T, ODD, FORCED>(kp, kp.geo, sl, rv, cells, face_bits, face_u, x, first);
@@ -912,7 +924,7 @@ __global__ void __launch_bounds__(kBlock)
   if (x >= N) return;
   const Slots<T> sl(g, kp.S);
   const Rows rb(rows, kp.geo.n_row_words);
-  const RowView rv{&rb, nullptr, 0u, 0, 0, 0u};
+  const RowView rv{&rb, nullptr, 0u, 0, 0, 0u, -1};
   macros_cell<T, ODD>(kp, kp.geo, sl, rv, cells, face_bits, face_u, x, first, rho_out, v_out, N);
 }
 
@@ -967,7 +979,7 @@ __global__ void __launch_bounds__(B)
     const uint32_t x = x0 + threadIdx.x;
     if (x >= d.cell_end) continue;
     const Rows rb(rows + d.rows_begin, d.geo.n_row_words);
-    const RowView rv{&rb, nullptr, 0u, 0, 0, 0u};
+    const RowView rv{&rb, nullptr, 0u, 0, 0, 0u, -1};
     step_cell<T, ODD, FORCED>(kp, d.geo, sl, rv, cells, face_bits, face_u, x, first);
   }
 }
@@ -984,7 +996,7 @@ __global__ void __launch_bounds__(kBlock)
   if (x >= d.cell_end) return;
   const Slots<T> sl(g, kp.S);
   const Rows rb(rows + d.rows_begin, d.geo.n_row_words);
-  const RowView rv{&rb, nullptr, 0u, 0, 0, 0u};
+  const RowView rv{&rb, nullptr, 0u, 0, 0, 0u, -1};
   macros_cell<T, ODD>(kp, d.geo, sl, rv, cells, face_bits, face_u, x, first, rho_out, v_out, (uint32_t)kp.p.n_cells);
 }
 
@@ -1351,7 +1363,7 @@ void launch_step_parity(hipStream_t s, KP kp, void *g, const int32_t *cells, con
                         const uint32_t *face_bits, void *face_u, int64_t t) {
   kp.sr_seed = sr_seed(kp.p, t);
   using R = typename St<T>::R;
-  const unsigned grid = grid_for(kp.p.n_cells);
+  const unsigned grid = grid_for(kp.p.n_cells, (t & 1) ? LBM3D_XCD_CHUNK_ODD : LBM3D_XCD_CHUNK_EVEN);
   if (t & 1)
     k_step<T, true, FORCED><<<grid, kBlock, 0, s>>>(kp, (T *)g, cells, rows, face_bits, (R *)face_u, t == 0);
   else

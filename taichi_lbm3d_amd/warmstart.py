@@ -42,7 +42,7 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def converge(solids, graph_steps=50, max_steps=50_000, tol=1e-6, init_fields=None, check_every=100):
+def converge(solids, graph_steps=50, max_steps=50_000, tol=1e-6, init_fields=None, check_every=50):
     """Run a batch to steady state (criterion over 500 steps, checked every
     `check_every`); returns (batch, steps, converged, wall_s, history of
     (t, wall_s, permeability per sample)), the history starting at t = 0."""

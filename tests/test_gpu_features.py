@@ -201,5 +201,7 @@ def test_stochastic_rounding_keeps_weak_forcing(gpu):
     err = {d: rel_l2(out[d][fl], out["f64"][fl]) for d in out if d != "f64"}
     print("velocity rel-L2 against FP64:", err)
     assert err["f16"] > 0.9
-    assert err["f16sr"] < 0.1 and err["bf16sr"] < 0.2
-    assert err["f16sr"] < 0.2 * err["f16"]
+    # unbiased rounding measures ~0.01 (FP16) and ~0.002 (BF16) here; a rounding
+    # biased in FP16's subnormal range (a carry into the next binade keeping the
+    # 2^-24 grid) measured 0.19
+    assert err["f16sr"] < 0.05 and err["bf16sr"] < 0.05
