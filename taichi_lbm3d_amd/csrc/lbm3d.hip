@@ -20,6 +20,7 @@
 #include <hip/hip_fp16.h>
 #include <hip/hip_bf16.h>
 
+#include <algorithm>
 #include <atomic>
 #include <cstdint>
 #include <cstdio>
@@ -214,25 +215,37 @@ __device__ __forceinline__ int wrap(int a, int n) { return a < 0 ? n - 1 : (a > 
 
 // Block -> cell-range map.  Workgroups are dealt to the 8 XCDs round robin
 // (hardware block h runs on XCD h % 8).  With C > 0 the map hands each XCD runs
-// of C consecutive blocks: within every group of 8C hardware blocks, XCD k
-// takes logical blocks kC .. kC+C-1, so blocks sharing cache lines of a slot
-// mostly share an L2 while all XCDs still sweep the range together.  The grid
-// is padded to a multiple of 8C blocks.
-template <uint32_t C, bool REVERSE = false>
-__device__ __forceinline__ uint32_t logical_block() {
-  const uint32_t h = blockIdx.x;
-  uint32_t b = h;
-  if constexpr (C > 0) {
-    const uint32_t r = h % (8u * C);
-    b = (h - r) + (r & 7u) * C + (r >> 3);
-  }
-  return REVERSE ? gridDim.x - 1u - b : b;
+// of C consecutive blocks: within every group of 8C blocks, XCD k takes logical
+// blocks kC .. kC+C-1, so blocks sharing cache lines of a slot mostly share an
+// L2 while all XCDs still sweep the range together.  The last, partial group
+// is split into 8 equal runs instead, so no XCD gets a longer tail.
+// xcd_block(C, k, j, nb): the j-th of XCD k's blocks among nb (nb past its last).
+__host__ __device__ __forceinline__ uint32_t xcd_block(uint32_t C, uint32_t k, uint32_t j, uint32_t nb) {
+  const uint32_t full = nb / (8u * C);  // whole groups
+  if (j < full * C) return (j / C) * (8u * C) + k * C + (j % C);
+  const uint32_t jt = j - full * C, ct = (nb - full * 8u * C + 7u) / 8u;
+  const uint32_t b = full * 8u * C + k * ct + jt;
+  return jt < ct && b < nb ? b : nb;
 }
 
-inline unsigned grid_for(int64_t n, int64_t chunk = 1) {
+// this workgroup's block among nb (nb: none); hardware block h is the
+// (h / 8)-th block of XCD h % 8
+template <uint32_t C, bool REVERSE = false>
+__device__ __forceinline__ uint32_t logical_block(uint32_t nb) {
+  const uint32_t h = blockIdx.x;
+  uint32_t b = h;
+  if constexpr (C > 0) b = xcd_block(C, h & 7u, h >> 3, nb);
+  if (b >= nb) return nb;
+  return REVERSE ? nb - 1u - b : b;
+}
+
+// grid of k_step for n cells: every XCD gets as many hardware blocks as its
+// longest list of logical blocks
+inline unsigned grid_for(int64_t n, int64_t chunk = 0) {
   const int64_t b = (n + kBlock - 1) / kBlock;
-  const int64_t per = 8 * (chunk > 0 ? chunk : 1);
-  return (unsigned)((b + per - 1) / per * per);
+  if (chunk <= 0) return (unsigned)((b + 7) & ~int64_t(7));
+  const int64_t per = 8 * chunk, full = b / per, tail = b - full * per;
+  return (unsigned)(full * per + 8 * ((tail + 7) / 8));
 }
 
 // ---------------------------------------------------------------------------
@@ -893,10 +906,12 @@ template <typename T, bool ODD, bool FORCED>
 __global__ void __launch_bounds__(kBlock, LBM3D_STEP_MINB)
     k_step(const KP kp, T *__restrict__ g, const int32_t *__restrict__ cells, const uint4 *__restrict__ rows,
            const uint32_t *__restrict__ face_bits, typename St<T>::R *__restrict__ face_u, int first) {
-  const uint32_t x0 = logical_block<ODD ? LBM3D_XCD_CHUNK_ODD : LBM3D_XCD_CHUNK_EVEN,
-                                    !ODD && LBM3D_REVERSE_EVEN>() * kBlock;
   const uint32_t n = (uint32_t)kp.p.n_cells;
-  if (x0 >= n) return;  // whole padding blocks of the grid
+  const uint32_t nb = (n + kBlock - 1) / kBlock;
+  const uint32_t lb = logical_block<ODD ? LBM3D_XCD_CHUNK_ODD : LBM3D_XCD_CHUNK_EVEN,
+                                    !ODD && LBM3D_REVERSE_EVEN>(nb);
+  if (lb >= nb) return;  // padding blocks of the grid
+  const uint32_t x0 = lb * kBlock;
   const uint32_t x = x0 + threadIdx.x;
   const Slots<T> sl(g, kp.S, kp.sr_seed);
   const Rows rb(rows, kp.geo.n_row_words);
