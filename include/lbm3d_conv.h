@@ -68,6 +68,15 @@ int lbm3d_conv3d_wgrad(void *stream, const uint16_t *x, const uint16_t *dy, floa
                        int32_t h, int32_t wd, int32_t cin, int32_t cout, int32_t ks, void *workspace,
                        size_t workspace_bytes);
 
+/* The surrogate's distance input: dist[v] = Euclidean distance from voxel v to
+ * the nearest solid voxel (solid[v] != 0) of its volume, with every volume
+ * periodic, exact up to `margin` voxels and capped there (also when a volume
+ * has no solid).  n volumes of d x h x w voxels, C order; `work` is a float
+ * buffer of the same size.  Three separable min-plus passes over offsets
+ * |o| <= margin; margin 1..1024. */
+int lbm3d_distance_periodic(void *stream, const int8_t *solid, float *dist, float *work, int32_t n, int32_t d,
+                            int32_t h, int32_t w, int32_t margin);
+
 #ifdef __cplusplus
 }
 #endif

@@ -132,6 +132,7 @@ SIGNATURES = {
     "lbm3d_conv3d_dgrad": (_int, [_vp, _vp, _vp, _vp, _i32, _i32, _i32, _i32, _i32, _i32, _i32]),
     "lbm3d_conv3d_wgrad_workspace_bytes": (_sz, [_i32, _i32, _i32, _i32, _i32, _i32, _i32]),
     "lbm3d_conv3d_wgrad": (_int, [_vp, _vp, _vp, _vp, _i32, _i32, _i32, _i32, _i32, _i32, _i32, _vp, _sz]),
+    "lbm3d_distance_periodic": (_int, [_vp, _vp, _vp, _vp, _i32, _i32, _i32, _i32, _i32]),
     # include/lbm3d_io.h
     "lbm3d_io_last_error": (ctypes.c_char_p, []),
     "lbm3d_read_geo_ascii": (_i64, [ctypes.c_char_p, _i32, _i32, _i32, _vp]),
@@ -169,7 +170,8 @@ class Lib:
     def call(self, name: str, *args):
         rc = getattr(self.cdll, name)(*args)
         if rc != 0:
-            msg = self.cdll.lbm3d_conv_last_error() if name.startswith("lbm3d_conv") else None
+            conv = name.startswith("lbm3d_conv") or name == "lbm3d_distance_periodic"
+            msg = self.cdll.lbm3d_conv_last_error() if conv else None
             detail = msg.decode() if msg else self.last_error()
             raise LbmError(f"{name} failed ({rc}): {detail}")
         return rc

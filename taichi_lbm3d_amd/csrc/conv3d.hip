@@ -24,9 +24,16 @@
 
 #include "lbm3d.h"
 #include "lbm3d_conv.h"
+#include "xcd.h"
 
 #ifndef LBM3D_CONV_TILED
 #define LBM3D_CONV_TILED 1  // 3x3x3 forward convolutions stage the input brick in LDS
+#endif
+#ifndef LBM3D_CONV_XCD_CHUNK
+#define LBM3D_CONV_XCD_CHUNK 64  // tiled forward: runs of consecutive bricks per XCD (shared halos hit L2)
+#endif
+#ifndef LBM3D_CONV_WLDS
+#define LBM3D_CONV_WLDS 1  // tiled forward: the chunk's weights staged in LDS (0: loaded per K step)
 #endif
 #ifndef LBM3D_CONV_MW
 #define LBM3D_CONV_MW 2  // 16-voxel fragments per wave in the forward kernel
@@ -235,27 +242,40 @@ __global__ void __launch_bounds__(256)
 
 // ---------------------------------------------------------------------------
 // Forward 3x3x3 convolution with the input tile in LDS.  A workgroup owns a
-// brick of 2 x 8 x 16 output voxels (z, y, x) of one volume and 16*NT output
-// channels.  For each chunk of 16 input channels it stages the brick's halo
-// (4 x 10 x 18 voxels) in LDS once; every tap then reads its A operands from
-// there instead of re-gathering them from memory.  One MFMA K step covers two
-// taps x 16 channels: lane group g takes tap 2s + g/2, channels 8*(g%2) .. +7.
-// Each wave computes 64 of the brick's voxels (4 fragments) x 16*NT channels.
+// brick of 2 x 8 x 16 output voxels (z, y, x) of one volume and 16 output
+// channels (blockIdx.y).  For each chunk of 16 input channels it stages the
+// brick's halo (4 x 10 x 18 voxels) and the chunk's weight fragments of all 14
+// K steps in LDS in one round trip, so the K loop itself only reads LDS and
+// issues MFMAs.  One K step covers two taps x
+// 16 channels: lane group g takes tap 2s + g/2, channels 8*(g%2) .. +7.  Each
+// wave computes 64 of the brick's voxels (4 fragments of 16).
 // ---------------------------------------------------------------------------
 constexpr int kBz = 2, kBy = 8, kBx = 16;                  // brick
 constexpr int kHz = kBz + 2, kHy = kBy + 2, kHx = kBx + 2;  // halo
 constexpr int kHalo = kHz * kHy * kHx;                     // 720 voxels
+constexpr int kSteps = 14;                                 // K steps per 16-channel chunk (28 tap slots)
 
-template <int NT>
+// halo-index offset of tap t (dz, dy, dx in -1..1)
+__host__ __device__ constexpr int tap_halo_offset(int t) {
+  return (t / 9 - 1) * kHy * kHx + ((t / 3) % 3 - 1) * kHx + (t % 3 - 1);
+}
+
 __global__ void __launch_bounds__(256)
     k_conv_fwd_tiled(const uint16_t *__restrict__ x, const uint16_t *__restrict__ w,
                      const float *__restrict__ bias, uint16_t *__restrict__ y, const Vol vol, int cin, int cout,
-                     int kpad, int relu, int nbz, int nby, int nbx) {
+                     int kpad, int relu, int nbz, int nby, int nbx, uint32_t nbricks) {
   __shared__ u32x4 halo[kHalo * 2];  // [voxel][16 channels] as two 16-byte halves
+#if LBM3D_CONV_WLDS
+  __shared__ u32x4 wl[kSteps][64];    // [K step][lane]: the lane's B fragment
+#endif
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int g = lane >> 4, r16 = lane & 15;
-  // brick coordinates
-  uint32_t bi = blockIdx.x;
+  const int half = g & 1, second = g >> 1;  // channel half and tap parity of this lane's K values
+  // brick coordinates (bricks x fastest, then y, z, volume)
+  uint32_t bi = LBM3D_CONV_XCD_CHUNK > 0
+                    ? xcd_block(LBM3D_CONV_XCD_CHUNK, blockIdx.x & 7u, blockIdx.x >> 3, nbricks)
+                    : blockIdx.x;
+  if (bi >= nbricks) return;  // padding block of the grid
   const int bx = (int)(bi % (uint32_t)nbx);
   bi /= (uint32_t)nbx;
   const int by = (int)(bi % (uint32_t)nby);
@@ -263,77 +283,92 @@ __global__ void __launch_bounds__(256)
   const int bz = (int)(bi % (uint32_t)nbz);
   const int bb = (int)(bi / (uint32_t)nbz);
   const int z0 = bz * kBz, y0 = by * kBy, x0 = bx * kBx;
-  const int co0 = blockIdx.y * 16 * NT;
-  // this lane's output voxels: brick-local index 64*wave + 16*f + r16
-  int lz[4], ly[4], lx[4];
+  const int co = blockIdx.y * 16 + r16;  // this lane's output channel in the B and C fragments
+  const int64_t vstride = (int64_t)vol.d * vol.h * vol.w;  // voxels per volume
+  const uint16_t *xv = x + (size_t)bb * vstride * cin;     // this brick's volume
+  // The A operand of voxel fragment f, tap t: halo entry 2 * (hc_f + T(t)) + half,
+  // hc_f = the halo index of this lane's voxel (brick-local 64*wave + 16*f + r16:
+  // x = r16, y = 4*(wave&1) + f, z = wave>>1) and T(t) the tap's halo offset.  As
+  // a byte address: ab_f + 32 T(t) with ab_f = 16 (2 hc_f + half).
+  uint32_t ab[4];
 #pragma unroll
   for (int f = 0; f < 4; ++f) {
-    const int lv = 64 * wave + 16 * f + r16;
-    lz[f] = lv / (kBy * kBx);
-    ly[f] = (lv / kBx) % kBy;
-    lx[f] = lv % kBx;
+    const int hc = (((wave >> 1) + 1) * kHy + (4 * (wave & 1) + f + 1)) * kHx + (r16 + 1);
+    ab[f] = (uint32_t)(16 * (2 * hc + half));
   }
-  f32x4 acc[4][NT];
+  f32x4 acc[4];
 #pragma unroll
-  for (int f = 0; f < 4; ++f)
-#pragma unroll
-    for (int t = 0; t < NT; ++t) acc[f][t] = f32x4{0.f, 0.f, 0.f, 0.f};
-  const int half = g & 1;
+  for (int f = 0; f < 4; ++f) acc[f] = f32x4{0.f, 0.f, 0.f, 0.f};
+  const char *hb = reinterpret_cast<const char *>(halo);
   for (int c0 = 0; c0 < cin; c0 += 16) {
-    __syncthreads();
-    for (int e = threadIdx.x; e < kHalo * 2; e += 256) {
-      const int hv = e >> 1, hh = e & 1;
-      const int hz = hv / (kHy * kHx), hy = (hv / kHx) % kHy, hx = hv % kHx;
+    __syncthreads();  // the previous chunk's LDS reads are done
+#if LBM3D_CONV_WLDS
+    for (int e = threadIdx.x; e < kSteps * 64; e += 256) {
+      const int st = e >> 6, ln = e & 63, lg = ln >> 4;
+      const int tap = 2 * st + (lg >> 1);
+      wl[st][ln] = tap < 27 ? ld16(w + (size_t)(blockIdx.y * 16 + (ln & 15)) * kpad + tap * cin + c0 + 8 * (lg & 1))
+                            : u32x4{0u, 0u, 0u, 0u};
+    }
+#endif
+    // halo: one voxel (its 16 channels, two 16-byte loads) per thread and pass
+    for (int hv = threadIdx.x; hv < kHalo; hv += 256) {
+      const int hz = hv / (kHy * kHx), r = hv - hz * (kHy * kHx);
+      const int hy = r / kHx, hx = r - hy * kHx;
       int zz = z0 + hz - 1, yy = y0 + hy - 1, xx = x0 + hx - 1;
       if (vol.periodic) {  // halo voxels past a face come from the opposite face
         zz = wrap1(zz, vol.d);
         yy = wrap1(yy, vol.h);
         xx = wrap1(xx, vol.w);
       }
-      u32x4 val = u32x4{0u, 0u, 0u, 0u};
-      if (zz >= 0 && zz < vol.d && yy >= 0 && yy < vol.h && xx >= 0 && xx < vol.w)
-        val = ld16(x + ((((int64_t)bb * vol.d + zz) * vol.h + yy) * vol.w + xx) * cin + c0 + 8 * hh);
-      halo[e] = val;
+      u32x4 lo = u32x4{0u, 0u, 0u, 0u}, hi = lo;
+      if ((unsigned)zz < (unsigned)vol.d && (unsigned)yy < (unsigned)vol.h && (unsigned)xx < (unsigned)vol.w) {
+        const uint16_t *src = xv + (((int64_t)zz * vol.h + yy) * vol.w + xx) * cin + c0;
+        lo = ld16(src);
+        hi = ld16(src + 8);
+      }
+      halo[2 * hv] = lo;
+      halo[2 * hv + 1] = hi;
     }
     __syncthreads();
-#pragma unroll 2
-    for (int tp = 0; tp < 28; tp += 2) {
-      const bool live = tp + (g >> 1) < 27;
-      const int tap = live ? tp + (g >> 1) : 26;  // the 28th slot of the last step is zero padding
-      const int dz = tap / 9 - 1, dy = (tap / 3) % 3 - 1, dx = tap % 3 - 1;
-      u32x4 a[4];
+#pragma unroll
+    for (int s = 0; s < kSteps; ++s) {
+      // taps 2s and 2s+1 (this lane: 2s + second); the 28th slot (s = 13,
+      // second half) is zero padding
+      constexpr int kMin = -(kHy * kHx + kHx + 1);  // smallest tap offset
+      const int t1 = 2 * s + 1 < 27 ? 2 * s + 1 : 26;
+      const uint32_t dsec = second ? (uint32_t)(32 * (tap_halo_offset(t1) - tap_halo_offset(2 * s))) : 0u;
+      const uint32_t imm = (uint32_t)(32 * (tap_halo_offset(2 * s) - kMin));
+#if LBM3D_CONV_WLDS
+      const bf16x8 bv = __builtin_bit_cast(bf16x8, wl[s][lane]);
+#else
+      const bf16x8 bv = __builtin_bit_cast(
+          bf16x8, 2 * s + second < 27 ? ld16(w + (size_t)co * kpad + (2 * s + second) * cin + c0 + 8 * half)
+                                      : u32x4{0u, 0u, 0u, 0u});
+#endif
 #pragma unroll
       for (int f = 0; f < 4; ++f) {
-        const int hv = ((lz[f] + dz + 1) * kHy + (ly[f] + dy + 1)) * kHx + (lx[f] + dx + 1);
-        a[f] = live ? halo[2 * hv + half] : u32x4{0u, 0u, 0u, 0u};
-      }
-#pragma unroll
-      for (int t = 0; t < NT; ++t) {
-        const uint16_t *wr = w + (size_t)(co0 + 16 * t + r16) * kpad + tap * cin + c0 + 8 * half;
-        const bf16x8 bv = __builtin_bit_cast(bf16x8, live ? ld16(wr) : u32x4{0u, 0u, 0u, 0u});
-#pragma unroll
-        for (int f = 0; f < 4; ++f)
-          acc[f][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, a[f]), bv, acc[f][t], 0, 0,
-                                                               0);
+        u32x4 a = *reinterpret_cast<const u32x4 *>(hb + (ab[f] + (uint32_t)(32 * kMin) + dsec) + imm);
+        if (s == kSteps - 1 && second) a = u32x4{0u, 0u, 0u, 0u};
+        acc[f] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, a), bv, acc[f], 0, 0, 0);
       }
     }
   }
-  // C layout: lane holds rows (voxels) 4g .. 4g+3 of each fragment, column (channel) r16
+  // C layout: lane holds rows (voxels) 4g .. 4g+3 of each fragment, column
+  // (channel) r16: voxel x = 4g + e, y = 4*(wave&1) + f, z = wave>>1 of the brick
+  const float bsum = bias ? bias[co] : 0.f;
+  const int oz = z0 + (wave >> 1), oy0 = y0 + 4 * (wave & 1), ox0 = x0 + 4 * g;
+  if (oz >= vol.d) return;
+  uint16_t *yv = y + (size_t)bb * vstride * cout + (((int64_t)oz * vol.h + oy0) * vol.w + ox0) * cout + co;
 #pragma unroll
-  for (int t = 0; t < NT; ++t) {
-    const int co = co0 + 16 * t + r16;
-    const float bsum = bias ? bias[co] : 0.f;
+  for (int f = 0; f < 4; ++f) {
+    if (oy0 + f >= vol.h) break;
 #pragma unroll
-    for (int f = 0; f < 4; ++f)
-#pragma unroll
-      for (int e = 0; e < 4; ++e) {
-        const int lv = 64 * wave + 16 * f + 4 * g + e;
-        const int oz = z0 + lv / (kBy * kBx), oy = y0 + (lv / kBx) % kBy, ox = x0 + lv % kBx;
-        if (oz >= vol.d || oy >= vol.h || ox >= vol.w) continue;
-        float val = acc[f][t][e] + bsum;
-        if (relu) val = val > 0.f ? val : 0.f;
-        y[((((size_t)bb * vol.d + oz) * vol.h + oy) * vol.w + ox) * cout + co] = to_bf16(val);
-      }
+    for (int e = 0; e < 4; ++e) {
+      if (ox0 + e >= vol.w) break;
+      float val = acc[f][e] + bsum;
+      if (relu) val = val > 0.f ? val : 0.f;
+      yv[((int64_t)f * vol.w + e) * cout] = to_bf16(val);
+    }
   }
 }
 
@@ -475,6 +510,41 @@ void wgrad_plan(int32_t n, int32_t d, int32_t h, int32_t wd, int32_t cout, int32
   vblock = (uint32_t)vb;
 }
 
+// ---------------------------------------------------------------------------
+// Periodic distance to solid (the surrogate's second input channel): the
+// separable squared-distance transform as three min-plus passes,
+//   g(v) = min over |d| <= M of f(v + d e_axis) + d^2  (coordinates wrapped),
+// starting from f = 0 on solid cells and 3M^2 + 1 elsewhere, then
+// dist = min(sqrt(f), M).  Exact for distances up to M; larger ones are capped.
+// Threads run along the innermost axis, so every pass reads coalesced rows.
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256)
+    k_dist_pass(const int8_t *__restrict__ solid, const float *__restrict__ f, float *__restrict__ g,
+                int64_t total, int d0, int d1, int d2, int axis, int margin, int last) {
+  const int64_t v = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (v >= total) return;
+  const int z = (int)(v % d2);
+  const int64_t r = v / d2;
+  const int y = (int)(r % d1);
+  const int x = (int)((r / d1) % d0);
+  const int n = axis == 0 ? d0 : (axis == 1 ? d1 : d2);
+  const int c = axis == 0 ? x : (axis == 1 ? y : z);
+  const int64_t stride = axis == 0 ? (int64_t)d1 * d2 : (axis == 1 ? (int64_t)d2 : 1);
+  const int64_t line = v - (int64_t)c * stride;  // this voxel's line at coordinate 0
+  const float big = (float)(3 * margin * margin + 1);
+  float best = 3.0e38f;
+  for (int d = -margin; d <= margin; ++d) {
+    int cc = (c + d) % n;
+    if (cc < 0) cc += n;
+    const int64_t u = line + (int64_t)cc * stride;
+    const float fv = solid ? (solid[u] != 0 ? 0.f : big) : f[u];
+    best = fminf(best, fv + (float)(d * d));
+  }
+  // the square root in FP64 rounded once to FP32: the distance an exact EDT
+  // (FP64) reports, cast to FP32
+  g[v] = last ? fminf((float)sqrt((double)best), (float)margin) : best;
+}
+
 }  // namespace
 
 extern "C" {
@@ -500,16 +570,10 @@ int lbm3d_conv3d_fwd(void *stream, const uint16_t *x, const uint16_t *w, const f
   if (ks == 3 && LBM3D_CONV_TILED) {
     const int nbz = (d + kBz - 1) / kBz, nby = (h + kBy - 1) / kBy, nbx = (wd + kBx - 1) / kBx;
     const int64_t bricks = (int64_t)n * nbz * nby * nbx;
-    if (bricks > (int64_t)(UINT32_MAX / 256u)) return cfail(LBM3D_ESIZE, "too many bricks for one launch");
-    if (cout % 64 == 0)
-      k_conv_fwd_tiled<4><<<dim3((unsigned)bricks, cout / 64), 256, 0, s>>>(x, w, bias, y, vol, cin, cout, kpad, relu,
-                                                                             nbz, nby, nbx);
-    else if (cout % 32 == 0)
-      k_conv_fwd_tiled<2><<<dim3((unsigned)bricks, cout / 32), 256, 0, s>>>(x, w, bias, y, vol, cin, cout, kpad, relu,
-                                                                             nbz, nby, nbx);
-    else
-      k_conv_fwd_tiled<1><<<dim3((unsigned)bricks, cout / 16), 256, 0, s>>>(x, w, bias, y, vol, cin, cout, kpad, relu,
-                                                                             nbz, nby, nbx);
+    const int64_t grid = xcd_grid(bricks, LBM3D_CONV_XCD_CHUNK);
+    if (grid > (int64_t)(UINT32_MAX / 256u)) return cfail(LBM3D_ESIZE, "too many bricks for one launch");
+    k_conv_fwd_tiled<<<dim3((unsigned)grid, cout / 16), 256, 0, s>>>(x, w, bias, y, vol, cin, cout, kpad, relu, nbz,
+                                                                   nby, nbx, (uint32_t)bricks);
     return ccheck("lbm3d_conv3d_fwd");
   }
   constexpr int MW = LBM3D_CONV_MW;
@@ -576,6 +640,23 @@ int lbm3d_conv3d_wgrad(void *stream, const uint16_t *x, const uint16_t *dy, floa
   const int64_t total = (int64_t)cout * kpad;
   k_wgrad_fold<<<(unsigned)((total + 255) / 256), 256, 0, s>>>(part, dw, parts * kWgWaves, taps, cin, cout, kpad);
   return ccheck("lbm3d_conv3d_wgrad/fold");
+}
+
+int lbm3d_distance_periodic(void *stream, const int8_t *solid, float *dist, float *work, int32_t n, int32_t d,
+                            int32_t h, int32_t w, int32_t margin) {
+  if (n < 1 || d < 1 || h < 1 || w < 1) return cfail(LBM3D_EINVAL, "empty volume");
+  if (margin < 1 || margin > 1024) return cfail(LBM3D_EINVAL, "margin must be 1..1024");
+  if (!solid || !dist || !work) return cfail(LBM3D_EINVAL, "NULL tensor");
+  const int64_t total = (int64_t)n * d * h * w;
+  hipStream_t s = (hipStream_t)stream;
+  const unsigned blocks = (unsigned)((total + 255) / 256);
+  // innermost axis from the solid mask into dist, then the middle axis into
+  // work and the outer axis back into dist (a pass cannot run in place: it
+  // reads values other threads overwrite)
+  k_dist_pass<<<blocks, 256, 0, s>>>(solid, nullptr, dist, total, d, h, w, 2, margin, 0);
+  k_dist_pass<<<blocks, 256, 0, s>>>(nullptr, dist, work, total, d, h, w, 1, margin, 0);
+  k_dist_pass<<<blocks, 256, 0, s>>>(nullptr, work, dist, total, d, h, w, 0, margin, 1);
+  return ccheck("lbm3d_distance_periodic");
 }
 
 }  // extern "C"

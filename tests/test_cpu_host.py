@@ -392,12 +392,39 @@ def test_winding_number_oracle_closed_meshes():
     np.testing.assert_allclose(w[far], (r[far] < 2.0).astype(float), atol=1e-9)
 
 
-def test_surrogate_distance_input_matches_scipy_edt():
-    """The surrogate's distance channel (separable min-plus transform in torch,
-    run here on the CPU) equals scipy's exact EDT on a wrap-padded copy."""
+def test_distance_oracle_matches_brute_force():
+    """The periodic distance oracle (scipy's exact EDT on a wrap-padded copy,
+    capped at the margin) equals a brute-force minimum over every solid cell and
+    every periodic image, on small grids, including one smaller than the margin
+    and one without solid cells."""
     from oracle.geo_ref import periodic_distance_edt
-    from taichi_lbm3d_amd.geometry import sphere_pack
-    from taichi_lbm3d_amd.surrogate import periodic_distance
-    for L, por in [(24, 0.3), (40, 0.6), (48, 0.9)]:
-        s = sphere_pack((L, L, L), porosity=por, rmin=3, rmax=6, seed=L, periodic=(True,) * 3)
-        np.testing.assert_array_equal(periodic_distance(s).numpy(), periodic_distance_edt(s))
+    rng = np.random.default_rng(7)
+    for shape, frac, margin in [((6, 5, 7), 0.1, 24), ((9, 4, 8), 0.05, 3), ((5, 5, 5), 0.0, 4)]:
+        s = (rng.random(shape) < frac).astype(np.int8)
+        want = np.full(shape, float(margin))
+        sol = np.argwhere(s != 0)
+        if len(sol):
+            idx = np.indices(shape).reshape(3, -1).T
+            d2 = np.full(len(idx), np.inf)
+            for p in sol:
+                for img in np.ndindex(3, 3, 3):
+                    q = p + (np.array(img) - 1) * np.array(shape)
+                    d2 = np.minimum(d2, ((idx - q) ** 2).sum(1))
+            want = np.minimum(np.sqrt(d2), margin).reshape(shape)
+        np.testing.assert_array_equal(periodic_distance_edt(s, margin), want.astype(np.float32))
+
+def test_inference_padding_reaches_multiples_of_eight():
+    """Non-wrapping inference pads every axis by at least BORDER cells on both
+    sides to a multiple of 8, for odd and even extents alike, and the periodic
+    pad keeps the volume at its place."""
+    import torch
+    from taichi_lbm3d_amd.surrogate import BORDER, inference_pads, periodic_pad
+    for shape in [(50, 50, 50), (51, 51, 51), (5, 6, 7), (8, 16, 24), (161, 3, 2)]:
+        pads = inference_pads(shape, wrap=False)
+        for n, (lo, hi) in zip(shape, pads):
+            assert lo == BORDER and hi >= BORDER and (n + lo + hi) % 8 == 0
+        t = torch.arange(int(np.prod(shape))).reshape(shape)
+        u = periodic_pad(t, pads)
+        assert torch.equal(u[BORDER:BORDER + shape[0], BORDER:BORDER + shape[1], BORDER:BORDER + shape[2]], t)
+        assert torch.equal(u[0, BORDER:BORDER + shape[1], BORDER:BORDER + shape[2]], t[(-BORDER) % shape[0]])
+    assert inference_pads((16, 16, 16), wrap=True) == [(0, 0)] * 3

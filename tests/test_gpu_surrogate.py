@@ -204,3 +204,24 @@ def test_unet_periodic_inference_is_translation_equivariant(gpu):
     with torch.no_grad():
         y0 = net(x)
     assert not torch.equal(y0, y)
+
+
+@pytest.mark.parametrize("shape,por,margin", [((24, 24, 24), 0.3, 24), ((40, 32, 48), 0.6, 24),
+                                              ((48, 48, 48), 0.9, 24), ((10, 12, 9), 0.5, 24),
+                                              ((20, 16, 24), 0.7, 5)])
+def test_distance_kernel_matches_oracle(gpu, shape, por, margin):
+    """lbm3d_distance_periodic equals the scipy-EDT oracle exactly, batched and
+    single, on cubic and ragged periodic packs (one smaller than the margin)."""
+    from oracle.geo_ref import periodic_distance_edt
+    from taichi_lbm3d_amd.geometry import sphere_pack
+    from taichi_lbm3d_amd.surrogate import periodic_distance
+    torch = _torch()
+    packs = [sphere_pack(shape, porosity=por, rmin=2, rmax=5, seed=11 + i, periodic=(True,) * 3) for i in range(2)]
+    batch = periodic_distance(torch.as_tensor(np.stack(packs)).to(gpu), margin=margin).cpu().numpy()
+    for i, s in enumerate(packs):
+        want = periodic_distance_edt(s, margin)
+        np.testing.assert_array_equal(periodic_distance(s, margin=margin, device=gpu).cpu().numpy(), want)
+        np.testing.assert_array_equal(batch[i], want)
+    empty = np.zeros(shape, np.int8)
+    np.testing.assert_array_equal(periodic_distance(empty, margin=margin, device=gpu).cpu().numpy(),
+                                  np.full(shape, float(margin), np.float32))
