@@ -39,7 +39,7 @@
 extern "C" {
 #endif
 
-#define LBM3D_ABI_VERSION 7
+#define LBM3D_ABI_VERSION 8
 
 /* storage types for populations; F32, F16 and BF16 compute in FP32, F64 in FP64 */
 #define LBM3D_F32 0
@@ -54,6 +54,12 @@ extern "C" {
  * per-step increments far below the 16-bit spacing, e.g. of a weak body
  * force, from rounding away.  No effect on F32/F64 storage. */
 #define LBM3D_FLAG_STOCHASTIC_ROUNDING 1
+/* lbm3d_params.flags: the row table given to the step functions is followed,
+ * at rows + lbm3d_row_words(p), by the lbm3d_tile_words(p) tile descriptors
+ * that lbm3d_build_tiles wrote for the same cell list.  Odd steps then take
+ * each block's rows from its descriptor and each cell's place from the row
+ * words instead of reading the cell list first.  Results are identical. */
+#define LBM3D_FLAG_TILES 2
 
 /* error codes */
 #define LBM3D_OK 0
@@ -106,6 +112,9 @@ const char *lbm3d_last_error(void);
 int64_t lbm3d_slot_stride(int32_t n_cells);
 /* Entries of the row table: nx * ny * ((nz + 31) / 32). */
 int64_t lbm3d_row_words(const lbm3d_params *p);
+/* Tile descriptors (lbm3d_row_word-sized entries) for p->n_cells fluid cells:
+ * one per block of cells of the step kernels; 0 when n_cells is 0. */
+int64_t lbm3d_tile_words(const lbm3d_params *p);
 /* uint32 words of face_bits: 2 bits per fluid cell, (n_cells + 15) / 16. */
 int64_t lbm3d_face_bit_words(int32_t n_cells);
 /* Nodes in the plane of face f (ny*nz, nx*nz or nx*ny).  The state of a
@@ -139,6 +148,13 @@ int lbm3d_build_rows(void *stream, const lbm3d_params *p, const int8_t *solid, c
  * the row table of one sample of a batch, whose cells start at id_offset. */
 int lbm3d_build_rows_at(void *stream, const lbm3d_params *p, const int8_t *solid, const int32_t *d2c,
                         int32_t id_offset, lbm3d_row_word *rows);
+
+/* Tile descriptors (LBM3D_FLAG_TILES) from the fluid list of pass 2, one per
+ * block of cells: {i, j, J, 0} when the block's cells lie in rows
+ * (i, j .. j+J) of one plane and their row window fits the odd step's LDS
+ * copy, else {i, j, 0xFFFFFFFF, 0}.  No reference counterpart: an index of the
+ * row table for the odd step. */
+int lbm3d_build_tiles(void *stream, const lbm3d_params *p, const int32_t *cells, lbm3d_row_word *tiles);
 
 /* face_bits: per fluid cell, bit 2*(x%16) of word x/16 set when the cell
  * belongs to a face in mode 1 or 2 (owner = last such face in face order),

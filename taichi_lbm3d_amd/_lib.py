@@ -15,9 +15,10 @@ LIB_PATH = os.path.join(_HERE, "liblbm3d.so")
 SOURCES = [os.path.join(_HERE, "csrc", name) for name in ("lbm3d.hip", "conv3d.hip", "geo_io.hip")]
 INCLUDE = os.path.join(_ROOT, "include")
 
-ABI_VERSION = 7
+ABI_VERSION = 8
 F32, F16, BF16, F64 = 0, 1, 2, 3
 FLAG_STOCHASTIC_ROUNDING = 1
+FLAG_TILES = 2
 # storage name -> storage type code; "...sr" names are the 16-bit types with
 # stochastic rounding (the flag bits of STORAGE_FLAGS set in lbm3d_params.flags)
 DTYPES = {"f32": F32, "f16": F16, "bf16": BF16, "f64": F64, "f16sr": F16, "bf16sr": BF16}
@@ -98,12 +99,14 @@ SIGNATURES = {
     "lbm3d_last_error": (ctypes.c_char_p, []),
     "lbm3d_slot_stride": (_i64, [_i32]),
     "lbm3d_row_words": (_i64, [_pp]),
+    "lbm3d_tile_words": (_i64, [_pp]),
     "lbm3d_face_bit_words": (_i64, [_i32]),
     "lbm3d_face_plane": (_i64, [_pp, _i32]),
     "lbm3d_compact_workspace_bytes": (_sz, [_i64]),
     "lbm3d_compact_count": (_int, [_vp, _pp, _vp, _vp, _vp, _sz]),
     "lbm3d_compact_fill": (_int, [_vp, _pp, _vp, _vp, _vp, _vp, _sz]),
     "lbm3d_build_rows": (_int, [_vp, _pp, _vp, _vp, _vp]),
+    "lbm3d_build_tiles": (_int, [_vp, _pp, _vp, _vp]),
     "lbm3d_build_rows_at": (_int, [_vp, _pp, _vp, _vp, _i32, _vp]),
     "lbm3d_build_face_bits": (_int, [_vp, _pp, _vp, _vp, _vp]),
     "lbm3d_build_neighbors": (_int, [_vp, _pp, _vp, _vp, _vp]),

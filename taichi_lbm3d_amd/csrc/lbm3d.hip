@@ -694,6 +694,7 @@ struct RowView {
   int i0, j0;
   uint32_t c0;  // dense index of cell (i0, j0, 0)
   int32_t cell; // >= 0: the cell's dense index, loaded ahead (else read from cells[])
+  int jrel, k;  // cell >= 0 and jrel >= 0: the cell's row in the window and k, found without cells[]
 };
 
 // Odd step: byte offsets off[q] of the addresses A_q within q's slot pair, and
@@ -704,14 +705,20 @@ __device__ __forceinline__ void odd_offsets(const Slots<T> &sl, const Geo &gm, c
   const uint32_t xb = x * (uint32_t)sizeof(T);
   cc.have_coords = true;
   int32_t nb[18];
-  const uint32_t dense = rv.cell >= 0 ? (uint32_t)rv.cell : (uint32_t)cells[x];
+  const uint32_t dense = rv.cell >= 0 ? (uint32_t)rv.cell : (rv.win && rv.jrel >= 0 ? 0u : (uint32_t)cells[x]);
   if (rv.win) {
     // the window's cells all lie in plane i0, rows j0 .. j0+J
-    const uint32_t d = dense - rv.c0;
-    const uint32_t jrel = fdiv(d, gm.m_nz, gm.s_nz);
+    uint32_t jrel;
+    if (rv.jrel >= 0) {
+      jrel = (uint32_t)rv.jrel;
+      cc.k = rv.k;
+    } else {
+      const uint32_t d = dense - rv.c0;
+      jrel = fdiv(d, gm.m_nz, gm.s_nz);
+      cc.k = (int)(d - jrel * (uint32_t)gm.nz);
+    }
     cc.i = rv.i0;
     cc.j = rv.j0 + (int)jrel;
-    cc.k = (int)(d - jrel * (uint32_t)gm.nz);
     const WindowRows src(rv.win, gm.W, rv.JW, (int)jrel, cc.k);
     neighbor_ids(src, gm, cc.k, nb);
   } else {
@@ -889,9 +896,77 @@ __device__ __forceinline__ RowView row_window(const Geo &gm, const int32_t *__re
       win[e] = rows[row * gm.W + w];
     }
     __syncthreads();
-    return RowView{&rb, win, JW, i0, j0, ((uint32_t)i0 * (uint32_t)gm.ny + (uint32_t)j0) * (uint32_t)gm.nz, -1};
+    return RowView{&rb, win, JW, i0, j0, ((uint32_t)i0 * (uint32_t)gm.ny + (uint32_t)j0) * (uint32_t)gm.nz, -1, -1, 0};
   }
-  return RowView{&rb, nullptr, 0u, 0, 0, 0u, -1};
+  return RowView{&rb, nullptr, 0u, 0, 0, 0u, -1, -1, 0};
+}
+
+// Tile descriptors (LBM3D_FLAG_TILES): per block of kBlock cells, {i0, j0, J,
+// 0} when the block's cells lie in rows (i0, j0 .. j0+J) of one plane, its row
+// window fits kWinWords and the middle plane's (J+1) W words fit one wave;
+// else {i0, j0, 0xFFFFFFFF, 0}.  The odd step then stages the window straight
+// from the descriptor and finds every cell's place from the window's words,
+// without reading the cell list.
+constexpr uint32_t kNoTile = 0xFFFFFFFFu;
+
+__device__ __forceinline__ RowView tile_window(const Geo &gm, const uint4 *__restrict__ rows, const Rows &rb,
+                                               const uint4 td, uint4 *win) {
+  const int i0 = (int)td.x, j0 = (int)td.y;
+  const uint32_t JW = td.z + 3u, per_plane = JW * gm.W, cnt = 3u * per_plane;
+  for (uint32_t e = threadIdx.x; e < cnt; e += kBlock) {
+    const uint32_t di = e / per_plane, r = e - di * per_plane;
+    const uint32_t dj = r / gm.W, w = r - dj * gm.W;
+    const uint32_t row =
+        (uint32_t)wrap(i0 + (int)di - 1, gm.nx) * (uint32_t)gm.ny + (uint32_t)wrap(j0 + (int)dj - 1, gm.ny);
+    win[e] = rows[row * gm.W + w];
+  }
+  __syncthreads();
+  return RowView{&rb, win, JW, i0, j0, ((uint32_t)i0 * (uint32_t)gm.ny + (uint32_t)j0) * (uint32_t)gm.nz, -1, -1, 0};
+}
+
+// position of the r-th (from 0) set bit of m (r < popcount(m))
+__device__ __forceinline__ uint32_t select_bit(uint32_t m, uint32_t r) {
+  uint32_t pos = 0u, c;
+  c = __popc(m & 0xFFFFu);
+  if (r >= c) { r -= c; m >>= 16; pos += 16u; }
+  c = __popc(m & 0xFFu);
+  if (r >= c) { r -= c; m >>= 8; pos += 8u; }
+  c = __popc(m & 0xFu);
+  if (r >= c) { r -= c; m >>= 4; pos += 4u; }
+  c = __popc(m & 0x3u);
+  if (r >= c) { r -= c; m >>= 2; pos += 2u; }
+  return pos + ((m & 1u) <= r ? 1u : 0u);
+}
+
+// Row (jrel, from j0) and k of cell x of the window's middle plane, from its
+// (J+1) W <= 64 words: lane l holds word l; an exclusive suffix minimum gives
+// empty words the first id of the next word with a fluid cell, and a binary
+// search over the lanes finds the last word whose first id is <= x.  Every
+// lane of the wave must call it.
+__device__ __forceinline__ void tile_place(const Geo &gm, const uint4 *win, uint32_t JW, uint32_t J, uint32_t x,
+                                           int &jrel, int &k) {
+  const uint32_t W = gm.W, nw = (J + 1u) * W, e0 = (JW + 1u) * W;
+  const uint32_t lane = threadIdx.x & 63u;
+  uint32_t key = 0x7FFFFFFFu;
+  if (lane < nw) {
+    const uint4 wd = win[e0 + lane];
+    if (wd.x) key = wd.y;
+  }
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    const uint32_t o = (uint32_t)__shfl_down((int)key, off);
+    if (lane + (uint32_t)off < 64u) key = min(key, o);
+  }
+  uint32_t pos = 0u;
+#pragma unroll
+  for (uint32_t step = 32u; step > 0u; step >>= 1) {
+    const uint32_t cand = pos + step;
+    const uint32_t kc = (uint32_t)__shfl((int)key, (int)(cand < 64u ? cand : 63u));
+    if (cand < nw && kc <= x) pos = cand;
+  }
+  const uint4 hw = win[e0 + pos];
+  k = (int)(32u * (pos % W) + select_bit(hw.x, x - hw.y));
+  jrel = (int)(pos / W);
 }
 
 template <typename T, bool ODD, bool FORCED>
@@ -907,14 +982,24 @@ __global__ void __launch_bounds__(kBlock, LBM3D_STEP_MINB)
   const uint32_t x = x0 + threadIdx.x;
   const Slots<T> sl(g, kp.S, kp.sr_seed);
   const Rows rb(rows, kp.geo.n_row_words);
-  RowView rv{&rb, nullptr, 0u, 0, 0, 0u, -1};
+  RowView rv{&rb, nullptr, 0u, 0, 0, 0u, -1, -1, 0};
   if constexpr (ODD && LBM3D_ROW_WINDOW) {
-    // the cell's dense index is loaded before the window is staged, so the two
-    // loads overlap instead of following each other
-    const int32_t own = cells[x < n ? x : n - 1];
     __shared__ uint4 win[kWinWords];
-    rv = row_window(kp.geo, cells, rows, rb, x0, n, win);
-    rv.cell = own;
+    if (kp.p.flags & LBM3D_FLAG_TILES) {
+      const uint4 td = rows[kp.geo.n_row_words + lb];  // this block's descriptor
+      if (td.z != kNoTile) {
+        rv = tile_window(kp.geo, rows, rb, td, win);
+        tile_place(kp.geo, win, rv.JW, td.z, x, rv.jrel, rv.k);
+      } else {
+        rv.cell = cells[x < n ? x : n - 1];  // table path
+      }
+    } else {
+      // the cell's dense index is loaded before the window is staged, so the
+      // two loads overlap instead of following each other
+      const int32_t own = cells[x < n ? x : n - 1];
+      rv = row_window(kp.geo, cells, rows, rb, x0, n, win);
+      rv.cell = own;
+    }
   }
   if (x >= n) return;
   step_cell<T, ODD, FORCED>(kp, kp.geo, sl, rv, cells, face_bits, face_u, x, first);
@@ -931,7 +1016,7 @@ __global__ void __launch_bounds__(kBlock)
   if (x >= N) return;
   const Slots<T> sl(g, kp.S);
   const Rows rb(rows, kp.geo.n_row_words);
-  const RowView rv{&rb, nullptr, 0u, 0, 0, 0u, -1};
+  const RowView rv{&rb, nullptr, 0u, 0, 0, 0u, -1, -1, 0};
   macros_cell<T, ODD>(kp, kp.geo, sl, rv, cells, face_bits, face_u, x, first, rho_out, v_out, N);
 }
 
@@ -986,7 +1071,7 @@ __global__ void __launch_bounds__(B)
     const uint32_t x = x0 + threadIdx.x;
     if (x >= d.cell_end) continue;
     const Rows rb(rows + d.rows_begin, d.geo.n_row_words);
-    const RowView rv{&rb, nullptr, 0u, 0, 0, 0u, -1};
+    const RowView rv{&rb, nullptr, 0u, 0, 0, 0u, -1, -1, 0};
     step_cell<T, ODD, FORCED>(kp, d.geo, sl, rv, cells, face_bits, face_u, x, first);
   }
 }
@@ -1003,7 +1088,7 @@ __global__ void __launch_bounds__(kBlock)
   if (x >= d.cell_end) return;
   const Slots<T> sl(g, kp.S);
   const Rows rb(rows + d.rows_begin, d.geo.n_row_words);
-  const RowView rv{&rb, nullptr, 0u, 0, 0, 0u, -1};
+  const RowView rv{&rb, nullptr, 0u, 0, 0, 0u, -1, -1, 0};
   macros_cell<T, ODD>(kp, d.geo, sl, rv, cells, face_bits, face_u, x, first, rho_out, v_out, (uint32_t)kp.p.n_cells);
 }
 
@@ -1135,6 +1220,21 @@ __global__ void __launch_bounds__(kBlock) k_build_rows(const Geo gm, const int8_
   const int32_t il = d2c[row0 + left], ir = d2c[row0 + right];
   rows[t] = make_uint4(mask, (uint32_t)base, (uint32_t)(il >= 0 ? il + id_offset : -1),
                        (uint32_t)(ir >= 0 ? ir + id_offset : -1));
+}
+
+// tile descriptors, one thread per block of kBlock cells
+__global__ void __launch_bounds__(kBlock) k_build_tiles(const Geo gm, const int32_t *__restrict__ cells, uint32_t n,
+                                                        uint4 *__restrict__ tiles) {
+  const uint32_t t = blockIdx.x * kBlock + threadIdx.x;
+  const uint32_t x0 = t * (uint32_t)kBlock;
+  if (x0 >= n) return;
+  const uint32_t xl = x0 + kBlock - 1 < n ? x0 + kBlock - 1 : n - 1;
+  int i0, j0, k0, i1, j1, k1;
+  coords(gm, (uint32_t)cells[x0], i0, j0, k0);
+  coords(gm, (uint32_t)cells[xl], i1, j1, k1);
+  const uint32_t J = (uint32_t)(j1 - j0);
+  const bool ok = i0 == i1 && 3u * (J + 3u) * gm.W <= (uint32_t)kWinWords && (J + 1u) * gm.W <= 64u;
+  tiles[t] = make_uint4((uint32_t)i0, (uint32_t)j0, ok ? J : kNoTile, 0u);
 }
 
 // face_bits: 16 cells per word, 2 bits per cell
@@ -1624,6 +1724,24 @@ int lbm3d_build_rows_at(void *stream, const lbm3d_params *p, const int8_t *solid
 int lbm3d_build_rows(void *stream, const lbm3d_params *p, const int8_t *solid, const int32_t *d2c,
                      lbm3d_row_word *rows) {
   return lbm3d_build_rows_at(stream, p, solid, d2c, 0, rows);
+}
+
+int64_t lbm3d_tile_words(const lbm3d_params *p) {
+  if (!p || p->n_cells <= 0) return 0;
+  return ((int64_t)p->n_cells + kBlock - 1) / kBlock;
+}
+
+int lbm3d_build_tiles(void *stream, const lbm3d_params *p, const int32_t *cells, lbm3d_row_word *tiles) {
+  int rc = check_params(p);
+  if (rc) return rc;
+  if ((rc = check_cells(p))) return rc;
+  if (p->n_cells == 0) return LBM3D_OK;
+  if (!cells || !tiles) return fail(LBM3D_EINVAL, "%s", "NULL argument to lbm3d_build_tiles");
+  const Geo gm = make_geo(p->nx, p->ny, p->nz, p->face_u_off);
+  const int64_t nt = lbm3d_tile_words(p);
+  k_build_tiles<<<(unsigned)((nt + kBlock - 1) / kBlock), kBlock, 0, (hipStream_t)stream>>>(
+      gm, cells, (uint32_t)p->n_cells, (uint4 *)tiles);
+  return check_launch("lbm3d_build_tiles");
 }
 
 int lbm3d_build_face_bits(void *stream, const lbm3d_params *p, const int8_t *solid, const int32_t *cells,

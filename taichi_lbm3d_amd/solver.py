@@ -154,6 +154,9 @@ class LB3D_Solver_Single_Phase:
         self.y = np.linspace(0, self.ny, self.ny)
         self.z = np.linspace(0, self.nz, self.nz)
         self._device = device
+        # odd steps locate their cells through tile descriptors (LBM3D_FLAG_TILES);
+        # results are identical either way
+        self.use_tiles = True
         self._initialised = False
         self._compiled = False  # parameters frozen by the first step() (SP kernels compile then)
         self._t = 0
@@ -308,8 +311,16 @@ class LB3D_Solver_Single_Phase:
             d2c = torch.empty(n_dense, dtype=torch.int32, device=dev)
             cells = torch.empty(max(p.n_cells, 1), dtype=torch.int32, device=dev)
             lib.call("lbm3d_compact_fill", s, _ref(p), _ptr(solid), _ptr(d2c), _ptr(cells), _ptr(ws), ws_bytes)
-            rows = torch.empty(max(4 * lib.size("lbm3d_row_words", _ref(p)), 4), dtype=torch.int32, device=dev)
+            # row table, then the odd step's tile descriptors (LBM3D_FLAG_TILES)
+            n_row = lib.size("lbm3d_row_words", _ref(p))
+            n_tile = lib.size("lbm3d_tile_words", _ref(p))
+            rows = torch.empty(max(4 * (n_row + n_tile), 4), dtype=torch.int32, device=dev)
             lib.call("lbm3d_build_rows", s, _ref(p), _ptr(solid), _ptr(d2c), _ptr(rows))
+            if n_tile and self.use_tiles:
+                lib.call("lbm3d_build_tiles", s, _ref(p), _ptr(cells), _ptr(rows[4 * n_row:]))
+                p.flags |= _lib.FLAG_TILES
+            else:
+                p.flags &= ~_lib.FLAG_TILES
             face_bits = torch.zeros(max(lib.size("lbm3d_face_bit_words", p.n_cells), 1), dtype=torch.int32,
                                     device=dev)
             lib.call("lbm3d_build_face_bits", s, _ref(p), _ptr(solid), _ptr(cells), _ptr(face_bits))
