@@ -872,23 +872,24 @@ __device__ __forceinline__ void init_cell(const KP &kp, const Geo &gm, const Slo
   }
 }
 
-// A workgroup's cells x0 .. x0+kBlock-1 (x0 < n) usually lie in a few
+// A workgroup's B cells x0 .. x0+B-1 (x0 < n) usually lie in a few
 // consecutive rows (i0, j0 .. j0+J) of one plane: stage the rows i0-1 .. i0+1 x
 // j0-1 .. j0+J+1 of the row table in LDS so each cell's 9 row words are LDS
 // reads, not global loads.  Falls back to the table (rv.win == nullptr) when
 // the cells span planes or the window would not fit.  Every thread of the
 // workgroup must call it (it synchronises).
+template <int B = kBlock>
 __device__ __forceinline__ RowView row_window(const Geo &gm, const int32_t *__restrict__ cells,
                                               const uint4 *__restrict__ rows, const Rows &rb, uint32_t x0,
                                               uint32_t n, uint4 *win) {
-  const uint32_t xl = x0 + kBlock - 1 < n ? x0 + kBlock - 1 : n - 1;
+  const uint32_t xl = x0 + B - 1 < n ? x0 + B - 1 : n - 1;
   int i0, j0, k0, i1, j1, k1;
   coords(gm, (uint32_t)cells[x0], i0, j0, k0);
   coords(gm, (uint32_t)cells[xl], i1, j1, k1);
   const uint32_t JW = (uint32_t)(j1 - j0) + 3u;
   if (i0 == i1 && JW * 3u * gm.W <= (uint32_t)kWinWords) {
     const uint32_t per_plane = JW * gm.W, cnt = 3u * per_plane;
-    for (uint32_t e = threadIdx.x; e < cnt; e += kBlock) {
+    for (uint32_t e = threadIdx.x; e < cnt; e += B) {
       const uint32_t di = e / per_plane, r = e - di * per_plane;
       const uint32_t dj = r / gm.W, w = r - dj * gm.W;
       const uint32_t row =
@@ -1056,24 +1057,72 @@ struct Batch {
 };
 
 // Persistent launch: gridDim.x workgroups walk the n_tiles tiles of B cells,
-// forwards or backwards.
+// forwards or backwards.  With C > 0 (a grid of 8 * per_xcd workgroups) each
+// XCD walks runs of C consecutive tiles (xcd.h); odd steps stage each tile's
+// row window in LDS, as k_step does.
 template <typename T, bool ODD, bool FORCED, int B>
 __global__ void __launch_bounds__(B)
     k_step_batch(const KP kp, const Batch bt, T *__restrict__ g, const int32_t *__restrict__ cells,
                  const uint4 *__restrict__ rows, const uint32_t *__restrict__ face_bits,
                  typename St<T>::R *__restrict__ face_u, int first, uint32_t n_tiles, int reverse) {
+  constexpr uint32_t C = ODD ? LBM3D_XCD_CHUNK_ODD : LBM3D_XCD_CHUNK_EVEN;
   const Slots<T> sl(g, kp.S, kp.sr_seed);
-  for (uint32_t it = blockIdx.x; it < n_tiles; it += gridDim.x) {
+  const uint32_t per_xcd = gridDim.x >> 3;
+  for (uint32_t i = 0;; ++i) {
+    uint32_t it;
+    if constexpr (C > 0) {
+      it = xcd_block(C, blockIdx.x & 7u, (blockIdx.x >> 3) + i * per_xcd, n_tiles);
+    } else {
+      it = blockIdx.x + i * gridDim.x;
+    }
+    if (it >= n_tiles) break;  // every condition below is uniform in the workgroup
     const uint32_t x0 = (reverse ? n_tiles - 1u - it : it) * (uint32_t)B;
     const int s = bt.chunk_sample[x0 / kChunk];
     if (bt.active && !bt.active[s]) continue;
     const SampleDesc &d = bt.desc[s];
+    if (x0 >= d.cell_end) continue;
     const uint32_t x = x0 + threadIdx.x;
-    if (x >= d.cell_end) continue;
     const Rows rb(rows + d.rows_begin, d.geo.n_row_words);
-    const RowView rv{&rb, nullptr, 0u, 0, 0, 0u, -1, -1, 0};
-    step_cell<T, ODD, FORCED>(kp, d.geo, sl, rv, cells, face_bits, face_u, x, first);
+    RowView rv{&rb, nullptr, 0u, 0, 0, 0u, -1, -1, 0};
+    if constexpr (ODD && LBM3D_ROW_WINDOW) {
+      __shared__ uint4 win[kWinWords];
+      const int32_t own = cells[x < d.cell_end ? x : d.cell_end - 1];
+      __syncthreads();  // the previous tile's window reads are done
+      rv = row_window<B>(d.geo, cells, rows + d.rows_begin, rb, x0, d.cell_end, win);
+      rv.cell = own;
+    }
+    if (x < d.cell_end) step_cell<T, ODD, FORCED>(kp, d.geo, sl, rv, cells, face_bits, face_u, x, first);
   }
+}
+
+// One tile per workgroup (the launch shapes with no persistent grid): the
+// same per-tile work as k_step_batch without its loop, which keeps the odd
+// step's registers at the single-grid kernel's count.
+template <typename T, bool ODD, bool FORCED, int B>
+__global__ void __launch_bounds__(B)
+    k_step_batch1(const KP kp, const Batch bt, T *__restrict__ g, const int32_t *__restrict__ cells,
+                  const uint4 *__restrict__ rows, const uint32_t *__restrict__ face_bits,
+                  typename St<T>::R *__restrict__ face_u, int first, uint32_t n_tiles, int reverse) {
+  constexpr uint32_t C = ODD ? LBM3D_XCD_CHUNK_ODD : LBM3D_XCD_CHUNK_EVEN;
+  uint32_t it = blockIdx.x;
+  if constexpr (C > 0) it = xcd_block(C, blockIdx.x & 7u, blockIdx.x >> 3, n_tiles);
+  if (it >= n_tiles) return;  // padding block of the grid
+  const uint32_t x0 = (reverse ? n_tiles - 1u - it : it) * (uint32_t)B;
+  const int s = bt.chunk_sample[x0 / kChunk];
+  if (bt.active && !bt.active[s]) return;
+  const SampleDesc &d = bt.desc[s];
+  if (x0 >= d.cell_end) return;
+  const uint32_t x = x0 + threadIdx.x;
+  const Slots<T> sl(g, kp.S, kp.sr_seed);
+  const Rows rb(rows + d.rows_begin, d.geo.n_row_words);
+  RowView rv{&rb, nullptr, 0u, 0, 0, 0u, -1, -1, 0};
+  if constexpr (ODD && LBM3D_ROW_WINDOW) {
+    __shared__ uint4 win[kWinWords];
+    const int32_t own = cells[x < d.cell_end ? x : d.cell_end - 1];
+    rv = row_window<B>(d.geo, cells, rows + d.rows_begin, rb, x0, d.cell_end, win);
+    rv.cell = own;
+  }
+  if (x < d.cell_end) step_cell<T, ODD, FORCED>(kp, d.geo, sl, rv, cells, face_bits, face_u, x, first);
 }
 
 template <typename T, bool ODD>
@@ -1555,15 +1604,21 @@ void launch_batch_parity(hipStream_t s, const KP &kp, const Batch &bt, void *g, 
                          const lbm3d_launch &c) {
   using R = typename St<T>::R;
   const uint32_t n_tiles = (uint32_t)kp.p.n_cells / (uint32_t)c.block;
-  uint32_t grid = n_tiles;
+  constexpr uint32_t C = ODD ? LBM3D_XCD_CHUNK_ODD : LBM3D_XCD_CHUNK_EVEN;
+  // XCD runs need a grid of 8 * per_xcd workgroups
+  uint32_t grid = C > 0 ? (uint32_t)xcd_grid(n_tiles, C) : n_tiles;
   if (c.grid_per_cu > 0) {
     const uint64_t g_max = (uint64_t)device_cus() * (uint64_t)c.grid_per_cu;
-    grid = g_max < n_tiles ? (uint32_t)g_max : n_tiles;
+    if (g_max < grid) grid = C > 0 ? (uint32_t)std::max<uint64_t>(8, g_max / 8 * 8) : (uint32_t)g_max;
   }
   const int rev = c.reverse ? 1 : 0;
-#define LBM3D_BATCH_LAUNCH(B)                                                                          \
-  k_step_batch<T, ODD, FORCED, B><<<grid, B, 0, s>>>(kp, bt, (T *)g, cells, rows, face_bits, (R *)face_u, first, \
-                                                     n_tiles, rev)
+#define LBM3D_BATCH_LAUNCH(B)                                                                             \
+  if (c.grid_per_cu > 0)                                                                                  \
+    k_step_batch<T, ODD, FORCED, B><<<grid, B, 0, s>>>(kp, bt, (T *)g, cells, rows, face_bits, (R *)face_u,  \
+                                                       first, n_tiles, rev);                               \
+  else                                                                                                    \
+    k_step_batch1<T, ODD, FORCED, B><<<grid, B, 0, s>>>(kp, bt, (T *)g, cells, rows, face_bits, (R *)face_u, \
+                                                        first, n_tiles, rev)
   switch (c.block) {
     case 128: LBM3D_BATCH_LAUNCH(128); break;
     case 512: LBM3D_BATCH_LAUNCH(512); break;

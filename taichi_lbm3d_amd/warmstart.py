@@ -132,14 +132,17 @@ class CropSampler:
         return x, torch.stack(ys), torch.stack(ms)
 
 
-def train(net, sampler, iters, batch, crop, lr=1e-3, log_every=50, border=sg.BORDER):
+def train(net, sampler, iters, batch, crop, lr=1e-3, log_every=50, border=sg.BORDER, rho_weight=1.0):
     """Adam from lr with a cosine decay to lr/100; masked MSE over the fluid cells
     at least `border` cells inside the crop (the network's zero padding at a
-    crop face stands where the true field goes on; inference pads periodically)."""
+    crop face stands where the true field goes on; inference pads periodically).
+    The density channel's squared error is weighted by rho_weight (the warm
+    start itself uses the velocity only)."""
     import math
     import torch
     opt = torch.optim.Adam(net.parameters(), lr=lr)
     losses = []
+    chw = torch.tensor([1.0, 1.0, 1.0, rho_weight], device=net.device)
     inner = torch.zeros((crop,) * 3, device=net.device)
     inner[border:crop - border, border:crop - border, border:crop - border] = 1.0
     t0 = time.perf_counter()
@@ -149,8 +152,8 @@ def train(net, sampler, iters, batch, crop, lr=1e-3, log_every=50, border=sg.BOR
         x, y, m = sampler.batch(batch, crop)
         m = m * inner
         out = net(x)
-        err = (out - y) ** 2 * m[..., None]
-        loss = err.sum() / (m.sum() * 4 + 1e-9)
+        err = (out - y) ** 2 * m[..., None] * chw
+        loss = err.sum() / (m.sum() * chw.sum() + 1e-9)
         opt.zero_grad(set_to_none=True)
         loss.backward()
         opt.step()
@@ -167,7 +170,7 @@ def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--train-samples", type=int, default=64)
     ap.add_argument("--train-seed", type=int, default=1000)
-    ap.add_argument("--iters", type=int, default=4000)
+    ap.add_argument("--iters", type=int, default=16000)
     ap.add_argument("--batch", type=int, default=2)
     ap.add_argument("--crop", type=int, default=128)
     ap.add_argument("--eval-samples", type=int, default=32)
@@ -175,6 +178,7 @@ def main(argv=None):
     ap.add_argument("--eval-seed", type=int, default=5000)
     ap.add_argument("--max-steps", type=int, default=50_000)
     ap.add_argument("--save-model", default="")
+    ap.add_argument("--rho-weight", type=float, default=0.1, help="weight of the density channel in the loss")
     ap.add_argument("--no-calibrate", dest="calibrate", action="store_false",
                     help="skip the velocity scale fitted on the training samples")
     ap.add_argument("--out", default="")
@@ -208,7 +212,7 @@ def main(argv=None):
     # 2. training
     net = sg.UNet3D(dev, seed=0)
     sampler = CropSampler(inputs, targets, solids, dev, seed=1)
-    curve, train_s = train(net, sampler, args.iters, args.batch, args.crop)
+    curve, train_s = train(net, sampler, args.iters, args.batch, args.crop, rho_weight=args.rho_weight)
     result["train"] = {"iters": args.iters, "batch": args.batch, "crop": args.crop, "wall_s": round(train_s, 1),
                        "loss_first": curve[0][1], "loss_last": curve[-1][1], "curve": curve}
     del sampler, inputs, targets
