@@ -127,21 +127,6 @@ def main():
     r = s.reduce()
     if r["nonfinite"]:
         raise FloatingPointError("non-finite fields after the timed run")
-    # per-parity launch times (informational): the even step reads and writes
-    # each cell's own slots, the odd step gathers from and scatters to neighbours
-    # a GPU-side wait first lets the host enqueue all 40 launches and events
-    # before the GPU reaches them, so no interval holds a host gap
-    ev = [torch.cuda.Event(enable_timing=True) for _ in range(41)]
-    torch.cuda._sleep(100_000_000)
-    ev[0].record()
-    for i in range(40):
-        s.step()
-        ev[i + 1].record()
-    torch.cuda.synchronize()
-    dts = [ev[i].elapsed_time(ev[i + 1]) * 1e3 for i in range(40)]
-    t_par = [(args.warmup + args.steps + i) % 2 for i in range(40)]
-    even_us = float(np.median([d for d, t in zip(dts, t_par) if t == 0]))
-    odd_us = float(np.median([d for d, t in zip(dts, t_par) if t == 1]))
     bpf = BYTES_PER_FLUP[args.dtype]
     achieved = n_fluid * bpf / (ms_per_step / 1e3) / 1e9
     traffic = None
@@ -192,8 +177,7 @@ def main():
                    "fluid_cells": n_fluid, "storage": args.dtype, "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                     "bytes_per_flup": bpf, "kernel": "k_step (even and odd launches alternate)",
-                     "even_us": round(even_us, 2), "odd_us": round(odd_us, 2)},
+                     "bytes_per_flup": bpf, "kernel": "k_step (even and odd launches alternate)"},
         "cpu_baseline": cpu,
         "steady": steady,
     }
