@@ -5,7 +5,12 @@ The batched step (lbm3d_batch_step) takes, per AA parity, a launch shape
 Results do not depend on the shape (the kernel is bit-identical for every
 shape), only the time does, so the tuner times real steps of the caller's own
 batch: every candidate runs a few (even, odd) step pairs, each step timed with
-HIP events on the solver's stream, and the fastest shape per parity wins.
+HIP events on the solver's stream, and the fastest shape per parity is kept.
+A step's speed also depends on the step before it: an even step that sweeps
+against the preceding odd step's direction starts on the slots that step wrote
+last, still in the last-level cache.  So the per-parity winners, their two
+alternating-direction variants and the default pair are then timed as pairs,
+and the fastest pair wins.
 The winners are cached per device, storage type, force switch and batch size
 (bucketed by powers of two) in ~/.cache/taichi_lbm3d_amd/launch.json;
 TAICHI_LBM3D_TUNE_CACHE=0 disables the cache.
@@ -56,11 +61,12 @@ def _store_cache(key: str, entry: dict) -> None:
         pass
 
 
-def time_shape(batch, shape, pairs: int = 3):
+def time_shape(batch, shape, pairs: int = 3, odd_shape=None):
     """Median (even_us, odd_us) of `pairs` step pairs with launch shape `shape`
-    for both parities; the batch advances by 2 * (pairs + 1) steps."""
+    for both parities (or `odd_shape` for odd steps); the batch advances by
+    2 * (pairs + 1) steps."""
     import torch
-    batch.set_launch(shape, shape)
+    batch.set_launch(shape, odd_shape or shape)
     if batch.steps_done % 2:
         batch.run(1)
     batch.run(2)  # first launches of this shape
@@ -94,9 +100,18 @@ def tune(batch, candidates=CANDIDATES, pairs: int = 3, use_cache: bool = True):
         table.append((tuple(shape), e, o))
     best_even = min(table, key=lambda r: r[1])[0]
     best_odd = min(table, key=lambda r: r[2])[0]
+    pairs_tried = []
+    for ev, od in {(best_even, best_odd),
+                   (best_even[:2] + (1,), best_odd[:2] + (0,)),
+                   (best_even[:2] + (0,), best_odd[:2] + (1,)),
+                   ((256, 0, 1), (256, 0, 0))}:
+        e, o = time_shape(batch, ev, pairs, odd_shape=od)
+        pairs_tried.append((ev, od, e + o))
+    best_even, best_odd, _ = min(pairs_tried, key=lambda r: r[2])
     batch.set_launch(best_even, best_odd)
     entry = {"even": list(best_even), "odd": list(best_odd),
-             "table": [[list(s), round(e, 2), round(o, 2)] for s, e, o in table]}
+             "table": [[list(s), round(e, 2), round(o, 2)] for s, e, o in table],
+             "pairs": [[list(e), list(o), round(t, 2)] for e, o, t in pairs_tried]}
     if use_cache:
         _store_cache(key, entry)
-    return {"even": best_even, "odd": best_odd, "table": table, "cached": False}
+    return {"even": best_even, "odd": best_odd, "table": table, "pairs": pairs_tried, "cached": False}

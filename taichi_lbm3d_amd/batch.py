@@ -172,6 +172,9 @@ class BatchSolver:
         self._t = 0
         self._graph = None
         d["active"].fill_(1)
+        # every sample active: the steps get no mask (NULL), which lets even
+        # steps load populations before looking their sample up
+        self._all_active = True
 
     def init_from_macros(self, fields):
         """Warm start: f = feq(rho, v) per sample from dense (rho (nx,ny,nz), v (nx,ny,nz,3))
@@ -213,7 +216,8 @@ class BatchSolver:
         d = self._dev
         _lib.lib().call("lbm3d_batch_run", self._stream(), _ref(self.p), _lib.DTYPES[self.dtype], _ptr(d["g"]),
                         _ptr(d["desc"]), self.n_samples, _ptr(d["cells"]), _ptr(d["rows"]), _ptr(d["face_bits"]),
-                        _ptr(d["face_u"]), _ptr(d["active"]), int(t0), int(nsteps), self._launch_arg())
+                        _ptr(d["face_u"]), None if self._all_active else _ptr(d["active"]), int(t0), int(nsteps),
+                        self._launch_arg())
 
     def set_launch(self, even, odd):
         """Launch shapes (block, grid_per_cu, reverse) of even and odd steps."""
@@ -345,6 +349,9 @@ class BatchSolver:
         torch = _torch()
         keep = ~np.asarray(mask, bool)
         self._dev["active"].copy_(torch.from_numpy(keep.astype(np.uint8)).to(self.device))
+        if self._all_active != bool(keep.all()):
+            self._all_active = bool(keep.all())
+            self._graph = None  # a captured graph holds the mask argument
 
     def run_to_steady(self, tol: float = 1e-6, interval: int = 500, max_steps: int = 50_000, graph_steps: int = 0,
                       callback=None, check_every: int = 0):

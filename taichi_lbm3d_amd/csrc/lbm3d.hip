@@ -1108,12 +1108,33 @@ __global__ void __launch_bounds__(B)
   if constexpr (C > 0) it = xcd_block(C, blockIdx.x & 7u, blockIdx.x >> 3, n_tiles);
   if (it >= n_tiles) return;  // padding block of the grid
   const uint32_t x0 = (reverse ? n_tiles - 1u - it : it) * (uint32_t)B;
+  const uint32_t x = x0 + threadIdx.x;
+  const Slots<T> sl(g, kp.S, kp.sr_seed);
+  if constexpr (!ODD) {
+    if (!bt.active) {
+      // With every sample active, an even step issues its population loads
+      // (the cell's own slots, which exist for every id of the batch, padding
+      // included) before the sample lookup, instead of after two dependent
+      // round trips for the chunk's sample and its descriptor.
+      using R = typename St<T>::R;
+      R v[kQ];
+      uint32_t off[kQ];
+      CellCtx cc;
+      const RowView none{nullptr, nullptr, 0u, 0, 0, 0u, -1, -1, 0};
+      read_cell<T, false>(sl, kp.geo, cells, none, x, v, off, cc);
+      const SampleDesc &d = bt.desc[bt.chunk_sample[x0 / kChunk]];
+      if (x >= d.cell_end) return;
+      collide_cell<R, FORCED>(kp, d.geo, cells, face_bits, face_u, x, first, v, cc);
+      const uint32_t xb = x * (uint32_t)sizeof(T);
+#pragma unroll
+      for (int q = 0; q < kQ; ++q) sl.st_slot(opp(q), xb, v[q]);
+      return;
+    }
+  }
   const int s = bt.chunk_sample[x0 / kChunk];
   if (bt.active && !bt.active[s]) return;
   const SampleDesc &d = bt.desc[s];
   if (x0 >= d.cell_end) return;
-  const uint32_t x = x0 + threadIdx.x;
-  const Slots<T> sl(g, kp.S, kp.sr_seed);
   const Rows rb(rows + d.rows_begin, d.geo.n_row_words);
   RowView rv{&rb, nullptr, 0u, 0, 0, 0u, -1, -1, 0};
   if constexpr (ODD && LBM3D_ROW_WINDOW) {

@@ -173,8 +173,11 @@ def test_autotune_picks_valid_shapes(gpu, tmp_path, monkeypatch):
     monkeypatch.setenv("TAICHI_LBM3D_TUNE_CACHE", "0")
     b, singles = batch_case("periodic_force", [(24, 24, 24), (30, 20, 16)], seed=1)
     res = autotune.tune(b, candidates=[(128, 0, 0), (256, 0, 1), (512, 2, 0)], pairs=2)
-    assert res["even"] in [(128, 0, 0), (256, 0, 1), (512, 2, 0)]
+    for shape in (res["even"], res["odd"]):
+        assert shape[0] in autotune.BLOCKS and shape[1] in autotune.GRID_PER_CU and shape[2] in (0, 1)
     assert len(res["table"]) == 3
+    # the chosen pair is the fastest of the pairs timed together
+    assert (res["even"], res["odd"]) == min(res["pairs"], key=lambda r: r[2])[:2]
     t = b.steps_done
     b.run(4)
     for s in singles:
