@@ -35,6 +35,12 @@
 #ifndef LBM3D_CONV_WLDS
 #define LBM3D_CONV_WLDS 1  // tiled forward: the chunk's weights staged in LDS (0: loaded per K step)
 #endif
+#ifndef LBM3D_CONV_WGRAD_TILED
+#define LBM3D_CONV_WGRAD_TILED 1  // weight gradients stage dy and x (with its halo) transposed in LDS
+#endif
+#ifndef LBM3D_CONV_WGRAD_PREFETCH
+#define LBM3D_CONV_WGRAD_PREFETCH 0  // 1: load the next brick while the current one's MFMAs run (3 waves/SIMD)
+#endif
 #ifndef LBM3D_CONV_MW
 #define LBM3D_CONV_MW 2  // 16-voxel fragments per wave in the forward kernel
 #endif
@@ -468,8 +474,210 @@ __global__ void __launch_bounds__(64 * kWgWaves)
 }
 
 // dw[co][tap*cin + ci] = sum over the partial sets, in order; padding columns 0
-__global__ void k_wgrad_fold(const float *__restrict__ part, float *__restrict__ dw, int nsets, int taps, int cin,
-                             int cout, int kpad) {
+// ---------------------------------------------------------------------------
+// Weight gradient with the operands in LDS:
+//   dw[co][tap*cin + ci] = sum over voxels v of dy[v][co] * x[v + off(tap)][ci].
+// A workgroup owns 16 output channels co0.., 16 input channels c0.. and a run
+// of bricks (2 x 8 x 16 voxels, as the forward kernel's).  Per brick it stages
+// dy transposed, dyT[co][voxel], and x transposed, xs[ci][...]: for 3x3x3
+// kernels x's halo as [ci][halo row][x], for 1x1x1 kernels the brick itself as
+// [ci][voxel].  For each K chunk of 32 brick voxels (two brick rows) a wave
+// reads one A fragment from dyT and, for each of its taps, one B fragment from
+// xs: the 8 halo voxels the chunk's voxels see through that tap, consecutive
+// in an xs row but offset by dx + 1 elements, so assembled from one or two
+// aligned LDS reads.  3x3x3: wave w accumulates taps w, w+4, ... over all
+// chunks.  1x1x1: wave w accumulates chunks 2w and 2w+1.  Each wave writes one
+// partial 16 x 16 block per tap; k_wgrad_fold sums the partials in a fixed
+// order.
+// ---------------------------------------------------------------------------
+constexpr int kXRow = 24;  // halo row in xs, bf16 (18 used; 48 bytes keep rows 16-byte aligned)
+constexpr int kXRows = kHz * kHy;  // 40 halo rows
+// channel strides of the LDS tiles, in bf16: 4 dwords modulo 32 banks, so the
+// 16 lanes reading 16 channels at one position hit distinct banks
+constexpr int kXChan = kXRows * kXRow + 8;  // 968
+constexpr int kDyRow = 256 + 8;             // 264
+constexpr int wgrad_sets(int ks) { return ks == 3 ? 1 : 4; }  // partial sets a workgroup writes
+
+template <int KS>
+__global__ void __launch_bounds__(256)
+    k_conv_wgrad_tiled(const uint16_t *__restrict__ x, const uint16_t *__restrict__ dy, float *__restrict__ part,
+                       const Vol vol, int cin, int cout, int nbz, int nby, int nbx, uint32_t nbricks,
+                       uint32_t per_block) {
+  constexpr int taps = KS * KS * KS;
+  constexpr int xchan = KS == 3 ? kXChan : kDyRow;
+  constexpr int nacc = KS == 3 ? 7 : 1;
+  __shared__ __align__(16) uint16_t dyT[16 * kDyRow];
+  __shared__ __align__(16) uint16_t xs[16 * xchan];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int g = lane >> 4, r16 = lane & 15;
+  const int nci = cin / 16;
+  const int co0 = (blockIdx.y / nci) * 16, c0 = (blockIdx.y % nci) * 16;
+  const uint32_t b0 = blockIdx.x * per_block;
+  const uint32_t b1 = b0 + per_block < nbricks ? b0 + per_block : nbricks;
+  const int64_t vstride = (int64_t)vol.d * vol.h * vol.w;
+  f32x4 acc[nacc];
+#pragma unroll
+  for (int i = 0; i < nacc; ++i) acc[i] = f32x4{0.f, 0.f, 0.f, 0.f};
+  // Global loads of one brick into registers: dy (and x for 1x1x1) of this
+  // thread's brick voxel, and for 3x3x3 up to three of x's halo voxels.
+  constexpr int nh = KS == 3 ? (kHalo + 255) / 256 : 0;
+  u32x4 rdy[2], rx[2], rh[nh > 0 ? nh : 1][2];
+  auto load_brick = [&](uint32_t br) {
+    uint32_t bi = br;
+    const int bx = (int)(bi % (uint32_t)nbx);
+    bi /= (uint32_t)nbx;
+    const int by = (int)(bi % (uint32_t)nby);
+    bi /= (uint32_t)nby;
+    const int bz = (int)(bi % (uint32_t)nbz);
+    const int bb = (int)(bi / (uint32_t)nbz);
+    const int z0 = bz * kBz, y0 = by * kBy, x0 = bx * kBx;
+    const u32x4 zero = u32x4{0u, 0u, 0u, 0u};
+    {
+      const int t = threadIdx.x;
+      const int z = z0 + (t >> 7), y = y0 + ((t >> 4) & 7), xx = x0 + (t & 15);
+      rdy[0] = rdy[1] = rx[0] = rx[1] = zero;
+      if (z < vol.d && y < vol.h && xx < vol.w) {
+        const size_t v = (size_t)bb * vstride + ((int64_t)z * vol.h + y) * vol.w + xx;
+        rdy[0] = ld16(dy + v * cout + co0);
+        rdy[1] = ld16(dy + v * cout + co0 + 8);
+        if (KS == 1) {
+          rx[0] = ld16(x + v * cin + c0);
+          rx[1] = ld16(x + v * cin + c0 + 8);
+        }
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < nh; ++q) {
+      const int hv = threadIdx.x + 256 * q;
+      rh[q][0] = rh[q][1] = zero;
+      const int hz = hv / (kHy * kHx), r = hv - hz * (kHy * kHx);
+      const int hy = r / kHx, hx = r - hy * kHx;
+      const int zz = z0 + hz - 1, yy = y0 + hy - 1, xx = x0 + hx - 1;
+      if (hv < kHalo && (unsigned)zz < (unsigned)vol.d && (unsigned)yy < (unsigned)vol.h &&
+          (unsigned)xx < (unsigned)vol.w) {
+        const uint16_t *src = x + ((size_t)bb * vstride + ((int64_t)zz * vol.h + yy) * vol.w + xx) * cin + c0;
+        rh[q][0] = ld16(src);
+        rh[q][1] = ld16(src + 8);
+      }
+    }
+  };
+  // registers -> LDS, transposed
+  auto store_brick = [&]() {
+    const int t = threadIdx.x;
+#pragma unroll
+    for (int c = 0; c < 8; ++c) {
+      dyT[c * kDyRow + t] = (uint16_t)(rdy[0][c >> 1] >> (16 * (c & 1)));
+      dyT[(8 + c) * kDyRow + t] = (uint16_t)(rdy[1][c >> 1] >> (16 * (c & 1)));
+      if (KS == 1) {
+        xs[c * xchan + t] = (uint16_t)(rx[0][c >> 1] >> (16 * (c & 1)));
+        xs[(8 + c) * xchan + t] = (uint16_t)(rx[1][c >> 1] >> (16 * (c & 1)));
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < nh; ++q) {
+      const int hv = threadIdx.x + 256 * q;
+      if (hv < kHalo) {
+        const int hz = hv / (kHy * kHx), r = hv - hz * (kHy * kHx);
+        const int hy = r / kHx, hx = r - hy * kHx;
+        const int e = (hz * kHy + hy) * kXRow + hx;
+#pragma unroll
+        for (int c = 0; c < 8; ++c) {
+          xs[c * xchan + e] = (uint16_t)(rh[q][0][c >> 1] >> (16 * (c & 1)));
+          xs[(8 + c) * xchan + e] = (uint16_t)(rh[q][1][c >> 1] >> (16 * (c & 1)));
+        }
+      }
+    }
+  };
+  if (LBM3D_CONV_WGRAD_PREFETCH) load_brick(b0);
+  for (uint32_t br = b0; br < b1; ++br) {
+    if (!LBM3D_CONV_WGRAD_PREFETCH) load_brick(br);
+    __syncthreads();  // the previous brick's LDS reads are done
+    store_brick();
+    __syncthreads();
+    // the next brick's loads are in flight while this one's MFMAs run
+    if (LBM3D_CONV_WGRAD_PREFETCH && br + 1 < b1) load_brick(br + 1);
+    if (KS == 1) {
+#pragma unroll
+      for (int q = 0; q < 2; ++q) {
+        const int k0 = 32 * (2 * wave + q) + 8 * g;
+        const u32x4 a = *reinterpret_cast<const u32x4 *>(&dyT[r16 * kDyRow + k0]);
+        const u32x4 b = *reinterpret_cast<const u32x4 *>(&xs[r16 * xchan + k0]);
+        acc[0] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, a), __builtin_bit_cast(bf16x8, b),
+                                                         acc[0], 0, 0, 0);
+      }
+    } else {
+#pragma unroll 2
+      for (int kc = 0; kc < 8; ++kc) {
+        // K values 8g .. 8g+7 of this chunk: brick voxels x = 8(g&1) .. +7 of
+        // row y = (2kc + g/2) % 8, z = kc / 4
+        const u32x4 a = *reinterpret_cast<const u32x4 *>(&dyT[r16 * kDyRow + 32 * kc + 8 * g]);
+        const int lz = kc >> 2, ly = (2 * kc + (g >> 1)) & 7, jx = 8 * (g & 1);
+#pragma unroll
+        for (int i = 0; i < nacc; ++i) {
+          const int t = wave + 4 * i;  // wave-uniform
+          if (t < taps) {
+            const int dz = t / 9 - 1, dyo = (t / 3) % 3 - 1, dx = t % 3 - 1;
+            const uint16_t *rowp = &xs[r16 * xchan + ((lz + dz + 1) * kHy + (ly + dyo + 1)) * kXRow + jx];
+            // halo x = jx + dx + 1 .. +7, shifted by dx + 1 elements from an aligned read
+            const u32x4 w0 = *reinterpret_cast<const u32x4 *>(rowp);
+            u32x4 b = w0;
+            if (dx >= 0) {
+              const uint32_t w4 = *reinterpret_cast<const uint32_t *>(rowp + 8);
+              if (dx == 0)
+                b = u32x4{__builtin_amdgcn_alignbyte(w0[1], w0[0], 2u), __builtin_amdgcn_alignbyte(w0[2], w0[1], 2u),
+                          __builtin_amdgcn_alignbyte(w0[3], w0[2], 2u), __builtin_amdgcn_alignbyte(w4, w0[3], 2u)};
+              else
+                b = u32x4{w0[1], w0[2], w0[3], w4};
+            }
+            acc[i] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, a),
+                                                             __builtin_bit_cast(bf16x8, b), acc[i], 0, 0, 0);
+          }
+        }
+      }
+    }
+  }
+  // C layout: lane holds rows (co) 4g .. 4g+3, column (ci) r16; part[set][tap][co][ci]
+  // with set = block (3x3x3) or 4 * block + wave (1x1x1)
+#pragma unroll
+  for (int i = 0; i < nacc; ++i) {
+    const int t = KS == 3 ? wave + 4 * i : 0;
+    const size_t set = KS == 3 ? blockIdx.x : (size_t)blockIdx.x * 4 + wave;
+    if (t >= taps) break;
+    float *dst = part + (set * taps + t) * cout * cin + (size_t)(co0 + 4 * g) * cin + c0 + r16;
+#pragma unroll
+    for (int e = 0; e < 4; ++e) dst[(size_t)e * cin] = acc[i][e];
+  }
+}
+
+// One pass of the tree fold of partial weight-gradient sets, in place: the
+// thread for (group q, element e) sums sets q*16*stride + j*stride, j < 16, at
+// element e into the group's first set.  Passes with stride 1, 16, 256, ...
+// leave the total in set 0, summed in an order fixed by the set count alone.
+__global__ void k_wgrad_fold_pass(float *__restrict__ part, int64_t set_size, int nsets, int stride) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t groups = (nsets + 16ll * stride - 1) / (16ll * stride);
+  if (i >= groups * set_size) return;
+  const int64_t q = i / set_size, e = i - q * set_size;
+  const int64_t first = q * 16 * stride;
+  float v[16];
+#pragma unroll
+  for (int j = 0; j < 16; ++j) {
+    const int64_t set = first + (int64_t)j * stride;
+    v[j] = set < nsets ? part[set * set_size + e] : 0.f;
+  }
+#pragma unroll
+  for (int w = 8; w >= 1; w >>= 1)
+#pragma unroll
+    for (int j = 0; j < w; ++j) v[j] += v[j + w];
+  part[first * set_size + e] = v[0];
+}
+
+// Sums nsets partial sets into dw: tree passes of k_wgrad_fold_pass while more
+// than 16 sets remain, then k_wgrad_fold over the survivors.
+int wgrad_fold(hipStream_t s, float *part, float *dw, int nsets, int taps, int cin, int cout, int kpad);
+
+__global__ void k_wgrad_fold(const float *__restrict__ part, float *__restrict__ dw, int nsets, int set_step, int taps,
+                             int cin, int cout, int kpad) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= (int64_t)cout * kpad) return;
   const int co = (int)(i / kpad), k = (int)(i % kpad);
@@ -478,9 +686,25 @@ __global__ void k_wgrad_fold(const float *__restrict__ part, float *__restrict__
     const int tap = k / cin, ci = k % cin;
     const size_t stride = (size_t)taps * cout * cin;
     const size_t off = ((size_t)tap * cout + co) * cin + ci;
-    for (int q = 0; q < nsets; ++q) s += part[q * stride + off];
+    for (int q = 0; q < nsets; ++q) s += part[(size_t)q * set_step * stride + off];
   }
   dw[i] = s;
+}
+
+int wgrad_fold(hipStream_t s, float *part, float *dw, int nsets, int taps, int cin, int cout, int kpad) {
+  const int64_t set_size = (int64_t)taps * cout * cin;
+  int stride = 1;
+  while (nsets > 16 * stride) {  // sets 0, stride, 2*stride, ... hold the partial sums
+    const int64_t groups = (nsets + 16ll * stride - 1) / (16ll * stride);
+    const int64_t threads = groups * set_size;
+    k_wgrad_fold_pass<<<(unsigned)((threads + 255) / 256), 256, 0, s>>>(part, set_size, nsets, stride);
+    if (int rc = ccheck("lbm3d_conv3d_wgrad/fold pass")) return rc;
+    stride *= 16;
+  }
+  const int64_t total = (int64_t)cout * kpad;
+  k_wgrad_fold<<<(unsigned)((total + 255) / 256), 256, 0, s>>>(part, dw, (nsets + stride - 1) / stride, stride, taps,
+                                                                cin, cout, kpad);
+  return ccheck("lbm3d_conv3d_wgrad/fold");
 }
 
 int check_conv(int32_t n, int32_t d, int32_t h, int32_t wd, int32_t cin, int32_t cout, int32_t ks) {
@@ -494,6 +718,21 @@ int check_conv(int32_t n, int32_t d, int32_t h, int32_t wd, int32_t cin, int32_t
 }
 
 constexpr int kWgTargetBlocks = 2048;
+
+// Tiled weight gradient: bricks per workgroup and workgroups along the brick
+// range, about kWgTargetBlocks workgroups in all.
+int wgrad_tiled_plan(int32_t n, int32_t d, int32_t h, int32_t wd, int32_t cin, int32_t cout, uint32_t &nbricks,
+                     uint32_t &per) {
+  const int64_t nb = (int64_t)n * ((d + kBz - 1) / kBz) * ((h + kBy - 1) / kBy) * ((wd + kBx - 1) / kBx);
+  const int64_t tiles = (int64_t)(cout / 16) * (cin / 16);
+  int64_t P = (kWgTargetBlocks + tiles - 1) / tiles;
+  if (P > nb) P = nb;
+  if (P < 1) P = 1;
+  const int64_t pb = (nb + P - 1) / P;
+  nbricks = (uint32_t)nb;
+  per = (uint32_t)pb;
+  return (int)((nb + pb - 1) / pb);
+}
 
 void wgrad_plan(int32_t n, int32_t d, int32_t h, int32_t wd, int32_t cout, int32_t ks, int &parts, uint32_t &vblock) {
   const int64_t vox = (int64_t)n * d * h * wd;
@@ -607,6 +846,11 @@ int lbm3d_conv3d_dgrad(void *stream, const uint16_t *dy, const uint16_t *wt, uin
 size_t lbm3d_conv3d_wgrad_workspace_bytes(int32_t n, int32_t d, int32_t h, int32_t wd, int32_t cin, int32_t cout,
                                           int32_t ks) {
   if (check_conv(n, d, h, wd, cin, cout, ks)) return 0;
+  if (LBM3D_CONV_WGRAD_TILED) {
+    uint32_t nbricks, per;
+    const int parts = wgrad_tiled_plan(n, d, h, wd, cin, cout, nbricks, per);
+    return (size_t)parts * wgrad_sets(ks) * ks * ks * ks * cout * cin * sizeof(float);
+  }
   int parts;
   uint32_t vb;
   wgrad_plan(n, d, h, wd, cout, ks, parts, vb);
@@ -621,12 +865,27 @@ int lbm3d_conv3d_wgrad(void *stream, const uint16_t *x, const uint16_t *dy, floa
   if (workspace_bytes < lbm3d_conv3d_wgrad_workspace_bytes(n, d, h, wd, cin, cout, ks))
     return cfail(LBM3D_EINVAL, "wgrad workspace too small");
   const Vol vol = make_vol(n, d, h, wd);
+  hipStream_t s = (hipStream_t)stream;
+  if (LBM3D_CONV_WGRAD_TILED) {
+    uint32_t nbricks, per;
+    const int parts = wgrad_tiled_plan(n, d, h, wd, cin, cout, nbricks, per);
+    const int nbz = (d + kBz - 1) / kBz, nby = (h + kBy - 1) / kBy, nbx = (wd + kBx - 1) / kBx;
+    const dim3 grid((unsigned)parts, (unsigned)((cout / 16) * (cin / 16)));
+    if (ks == 3)
+      k_conv_wgrad_tiled<3>
+          <<<grid, 256, 0, s>>>(x, dy, (float *)workspace, vol, cin, cout, nbz, nby, nbx, nbricks, per);
+    else
+      k_conv_wgrad_tiled<1>
+          <<<grid, 256, 0, s>>>(x, dy, (float *)workspace, vol, cin, cout, nbz, nby, nbx, nbricks, per);
+    if ((rc = ccheck("lbm3d_conv3d_wgrad"))) return rc;
+    const int kpad = lbm3d_conv_kpad(cin, ks);
+    return wgrad_fold(s, (float *)workspace, dw, parts * wgrad_sets(ks), ks * ks * ks, cin, cout, kpad);
+  }
   int parts;
   uint32_t vb;
   wgrad_plan(n, d, h, wd, cout, ks, parts, vb);
   const int taps = ks * ks * ks;
   const size_t lds = (size_t)kWgWaves * (16 + cin) * kLd * sizeof(uint16_t);
-  hipStream_t s = (hipStream_t)stream;
   const dim3 grid((unsigned)parts, (unsigned)taps, (unsigned)(cout / 16));
   float *part = (float *)workspace;
   if (cin <= 64)
@@ -638,7 +897,7 @@ int lbm3d_conv3d_wgrad(void *stream, const uint16_t *x, const uint16_t *dy, floa
   if ((rc = ccheck("lbm3d_conv3d_wgrad"))) return rc;
   const int kpad = lbm3d_conv_kpad(cin, ks);
   const int64_t total = (int64_t)cout * kpad;
-  k_wgrad_fold<<<(unsigned)((total + 255) / 256), 256, 0, s>>>(part, dw, parts * kWgWaves, taps, cin, cout, kpad);
+  k_wgrad_fold<<<(unsigned)((total + 255) / 256), 256, 0, s>>>(part, dw, parts * kWgWaves, 1, taps, cin, cout, kpad);
   return ccheck("lbm3d_conv3d_wgrad/fold");
 }
 

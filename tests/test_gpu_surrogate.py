@@ -110,7 +110,9 @@ def test_conv_data_gradient_matches_cpu_autograd(gpu, shape):
     assert err <= 2 ** -8 * ref.abs().max().item() + 1e-6, err
 
 
-@pytest.mark.parametrize("shape", SHAPES)
+# the last two shapes leave hundreds of partial sets to fold (and give each 3x3x3
+# workgroup two bricks, the last one a single brick)
+@pytest.mark.parametrize("shape", SHAPES + [(1, 42, 72, 100, 16, 32, 3), (2, 20, 24, 40, 32, 16, 1)])
 def test_conv_weight_gradient_matches_cpu_autograd(gpu, shape):
     torch = _torch()
     from taichi_lbm3d_amd.surrogate import conv_wgrad
