@@ -3,13 +3,15 @@
     python taichi_lbm3d_amd/profile_unet.py [--size 128] [--batch 2] [--iters 20]
     python taichi_lbm3d_amd/profile_unet.py --infer [--size 192] [--batch 2] [--iters 20]
     python taichi_lbm3d_amd/profile_unet.py --layers [--size 192] [--batch 2] [--iters 10]
+    python taichi_lbm3d_amd/profile_unet.py --wgrad-layers [--size 128] [--batch 2] [--iters 10]
 
 Random inputs and targets of the training shape (batch x size^3, 2 input and
 4 output channels); each iteration is one forward, backward and Adam step,
 as in warmstart.train.  With --infer each iteration is one forward pass with
 periodic convolutions and no autograd, as in surrogate.predict.  With
 --layers every convolution of the network is timed alone at its inference
-shape (one JSON line per layer: time, TFLOP/s, activation GB/s).  Prints the
+shape (one JSON line per layer: time, TFLOP/s, activation GB/s); with
+--wgrad-layers every weight gradient at its training shape.  Prints the
 mean iteration time and the convolution FLOP rate it implies (GEMM FLOPs of
 every 3x3x3 and 1x1x1 convolution: forward, plus data and weight gradients
 when training).
@@ -72,6 +74,34 @@ def time_layers(args):
     print(json.dumps({"layers_total_us": round(total, 1)}))
 
 
+def time_wgrad_layers(args):
+    """Each convolution's weight gradient alone at its training shape
+    (batch x (size >> level)^3), random x and output gradient."""
+    import torch
+    from taichi_lbm3d_amd import surrogate as sg
+    dev = torch.device("cuda", 0)
+    net = sg.UNet3D(dev, seed=0)
+    total = 0.0
+    for name, (cin, cout, ks) in net.specs.items():
+        n = args.size >> LEVEL[name[:2]]
+        x = (torch.randn((args.batch, n, n, n, cin), device=dev) * 0.5).to(torch.bfloat16)
+        gy = (torch.randn((args.batch, n, n, n, cout), device=dev) * 0.5).to(torch.bfloat16)
+        for _ in range(3):
+            sg.conv_wgrad(x, gy, ks)
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(args.iters):
+            sg.conv_wgrad(x, gy, ks)
+        e1.record()
+        torch.cuda.synchronize()
+        us = e0.elapsed_time(e1) / args.iters * 1e3
+        total += us
+        print(json.dumps({"wgrad_layer": name, "cin": cin, "cout": cout, "edge": n, "us": round(us, 1),
+                          "tflops": round(2.0 * args.batch * n ** 3 * ks ** 3 * cin * cout / us / 1e6, 1)}),
+              flush=True)
+    print(json.dumps({"wgrad_layers_total_us": round(total, 1)}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--size", type=int, default=None, help="volume edge (default 128, or 192 with --infer)")
@@ -79,10 +109,14 @@ def main():
     ap.add_argument("--iters", type=int, default=20)
     ap.add_argument("--infer", action="store_true", help="time periodic forward passes instead of training steps")
     ap.add_argument("--layers", action="store_true", help="time each convolution alone at its inference shape")
+    ap.add_argument("--wgrad-layers", action="store_true",
+                    help="time each weight gradient alone at its training shape")
     args = ap.parse_args()
     args.size = args.size or (192 if (args.infer or args.layers) else 128)
     if args.layers:
         return time_layers(args)
+    if args.wgrad_layers:
+        return time_wgrad_layers(args)
     import torch
     from taichi_lbm3d_amd import surrogate as sg
     dev = torch.device("cuda", 0)
