@@ -276,22 +276,29 @@ def make_targets(rho, v, solid, fx, nu_eff):
     return np.where(fluid, np.concatenate([vt, rt], axis=-1), 0.0).astype(np.float32)
 
 
-def fields_from_prediction(pred, fluid, fx, nu_eff):
+def fields_from_prediction(pred, fluid, fx, nu_eff, density=False):
     """Inverse of make_targets on the device: (rho, v, predicted rho) from a
     (X,Y,Z,4) prediction, zero velocity off the fluid.
 
-    The warm start takes the predicted velocity with rho = 1, the cold start's
-    density.  Every isolated pore keeps its initial mass for good, and the
-    steady velocity of a pore scales as 1/rho, so a predicted density would pin
-    each pore's mean density -- and the permeability, to about the prediction's
-    density error -- away from the cold start's.  The predicted density is
-    returned for error reporting only."""
+    By default the warm start takes the predicted velocity with rho = 1, the
+    cold start's density.  Every isolated pore keeps its initial mass for good,
+    and the steady velocity of a pore scales as 1/rho, so a raw predicted
+    density would pin the mean density -- and the permeability, to about the
+    prediction's density error -- away from the cold start's.  With
+    density=True the returned rho is the predicted density shifted to mean 1
+    over the fluid cells: the sample's total mass is the cold start's, and only
+    the split of that mass between disconnected pores can differ."""
     torch = _torch()
     p = pred.float()
     fl = fluid.to(p.dtype)
     v = p[..., :3] * (fx / nu_eff) * fl[..., None]
     rho_pred = 1.0 + p[..., 3] * (fx * RHO_SCALE) * fl
-    return torch.ones_like(rho_pred), v, rho_pred
+    if not density:
+        return torch.ones_like(rho_pred), v, rho_pred
+    n = fl.sum().clamp_min(1.0)
+    dev = (rho_pred - 1.0) * fl
+    rho = 1.0 + (dev - dev.sum() / n) * fl
+    return rho, v, rho_pred
 
 
 def to_network_input(inp):
@@ -331,9 +338,11 @@ def _device_solid(solid, device):
     return torch.from_numpy(np.ascontiguousarray(np.asarray(solid) != 0)).to(device)
 
 
-def predict(net, solids, fx, nu_eff, batch: int = 2, wrap=None):
-    """Warm-start (rho = 1, v) device tensors predicted for periodic geometries of
+def predict(net, solids, fx, nu_eff, batch: int = 2, wrap=None, density=False):
+    """Warm-start (rho, v) device tensors predicted for periodic geometries of
     one shape (numpy arrays or device tensors), `batch` volumes per forward pass.
+    rho is 1, or with density=True the predicted density at mean 1 over the
+    fluid (fields_from_prediction).
 
     wrap=True runs the network's convolutions periodically on the volume itself
     (every extent a multiple of 8): every cell sees its true periodic
@@ -361,5 +370,5 @@ def predict(net, solids, fx, nu_eff, batch: int = 2, wrap=None):
             (X, Y, Z), (px, py, pz) = shape, (p[0] for p in pads)
             for i in range(sol.shape[0]):
                 core = pred[i, px:px + X, py:py + Y, pz:pz + Z]
-                out.append(fields_from_prediction(core, ~sol[i], fx, nu_eff)[:2])
+                out.append(fields_from_prediction(core, ~sol[i], fx, nu_eff, density)[:2])
     return out

@@ -178,9 +178,12 @@ def main(argv=None):
     ap.add_argument("--eval-seed", type=int, default=5000)
     ap.add_argument("--max-steps", type=int, default=50_000)
     ap.add_argument("--save-model", default="")
+    ap.add_argument("--load-model", default="", help="skip training: load the network from this file")
     ap.add_argument("--rho-weight", type=float, default=0.1, help="weight of the density channel in the loss")
     ap.add_argument("--no-calibrate", dest="calibrate", action="store_false",
                     help="skip the velocity scale fitted on the training samples")
+    ap.add_argument("--density", action="store_true",
+                    help="start from the predicted density (at mean 1 per sample) instead of rho = 1")
     ap.add_argument("--out", default="")
     args = ap.parse_args(argv)
 
@@ -211,11 +214,16 @@ def main(argv=None):
 
     # 2. training
     net = sg.UNet3D(dev, seed=0)
-    sampler = CropSampler(inputs, targets, solids, dev, seed=1)
-    curve, train_s = train(net, sampler, args.iters, args.batch, args.crop, rho_weight=args.rho_weight)
-    result["train"] = {"iters": args.iters, "batch": args.batch, "crop": args.crop, "wall_s": round(train_s, 1),
-                       "loss_first": curve[0][1], "loss_last": curve[-1][1], "curve": curve}
-    del sampler, inputs, targets
+    if args.load_model:
+        net.load_state_dict(torch.load(args.load_model, map_location=dev, weights_only=True))
+        result["train"] = {"loaded": args.load_model}
+    else:
+        sampler = CropSampler(inputs, targets, solids, dev, seed=1)
+        curve, train_s = train(net, sampler, args.iters, args.batch, args.crop, rho_weight=args.rho_weight)
+        result["train"] = {"iters": args.iters, "batch": args.batch, "crop": args.crop, "wall_s": round(train_s, 1),
+                           "loss_first": curve[0][1], "loss_last": curve[-1][1], "curve": curve}
+        del sampler
+    del inputs, targets
     torch.cuda.empty_cache()
     # 3. velocity scale: the training samples' permeability against the
     # surrogate's own (from its warm start at t = 0), median ratio
@@ -244,7 +252,7 @@ def main(argv=None):
     ev_dev = [torch.from_numpy(np.ascontiguousarray(s)).to(dev) for s in ev]
     torch.cuda.synchronize()
     t_inf = time.perf_counter()
-    preds = sg.predict(net, ev_dev, FX, nu)
+    preds = sg.predict(net, ev_dev, FX, nu, density=args.density)
     if scale != 1.0:
         preds = [(r, v * scale) for r, v in preds]
     torch.cuda.synchronize()
@@ -252,11 +260,14 @@ def main(argv=None):
     fallback = np.array([not (torch.isfinite(r).all() and torch.isfinite(v).all()) for r, v in preds])
     preds = [(torch.ones_like(r), torch.zeros_like(v)) if fb else (r, v) for (r, v), fb in zip(preds, fallback)]
     # the surrogate's error, judged by the converged cold-start fields
-    v_err = []
+    v_err, rho_err = [], []
     for (r, v), (rr, vr), s in zip(preds, ref, ev):
         fl = s == 0
         vp = v.cpu().numpy()
         v_err.append(float(np.linalg.norm(vp[fl] - vr[fl]) / np.linalg.norm(vr[fl])))
+        # density deviation from the sample's fluid mean, against the cold start's
+        dr, dp = rr[fl] - rr[fl].mean(), r.cpu().numpy()[fl] - 1.0
+        rho_err.append(float(np.linalg.norm(dp - dr) / np.linalg.norm(dr)))
     try:
         bw, steps_w, conv_w, wall_w, hist_w = converge(ev, max_steps=args.max_steps, init_fields=preds)
         k_warm = bw.permeability()
@@ -276,6 +287,7 @@ def main(argv=None):
     result["eval"] = {
         "samples": len(ev), "size": args.eval_size, "fallbacks": int(fallback.sum()),
         "surrogate": {"v_rel_l2_median": float(np.median(v_err)), "v_rel_l2_max": float(np.max(v_err)),
+                      "rho_dev_rel_l2_median": float(np.median(rho_err)),
                       "k_rel_err_median": float(np.median(perr)), "k_rel_err_max": float(np.max(perr)),
                       "k_signed_err_median": float(np.median(signed)), "velocity_scale": scale,
                       "inference_s": round(inf_s, 3)},

@@ -428,3 +428,30 @@ def test_inference_padding_reaches_multiples_of_eight():
         assert torch.equal(u[BORDER:BORDER + shape[0], BORDER:BORDER + shape[1], BORDER:BORDER + shape[2]], t)
         assert torch.equal(u[0, BORDER:BORDER + shape[1], BORDER:BORDER + shape[2]], t[(-BORDER) % shape[0]])
     assert inference_pads((16, 16, 16), wrap=True) == [(0, 0)] * 3
+
+
+def test_prediction_fields_keep_the_cold_start_mass():
+    """fields_from_prediction inverts make_targets on the fluid; the default
+    density is 1, and the predicted density (density=True) is shifted so the
+    fluid's total mass is the cold start's, one per fluid cell."""
+    import torch
+    from taichi_lbm3d_amd.surrogate import fields_from_prediction, make_targets
+    rng = np.random.default_rng(3)
+    solid = (rng.random((6, 5, 4)) < 0.3).astype(np.int8)
+    fx, nu = 1e-5, 1.0 / 6.0
+    v = rng.normal(0.0, 1e-4, (6, 5, 4, 3)).astype(np.float32)
+    rho = (1.0 + rng.normal(0.0, 3e-4, (6, 5, 4))).astype(np.float32)
+    pred = torch.from_numpy(make_targets(rho, v, solid, fx, nu))
+    fluid = torch.from_numpy(solid == 0)
+    r1, v1, rp = fields_from_prediction(pred, fluid, fx, nu)
+    assert torch.all(r1 == 1.0)
+    fl = solid == 0
+    np.testing.assert_allclose(v1.numpy()[fl], v[fl], rtol=1e-5, atol=1e-12)
+    assert np.all(v1.numpy()[~fl] == 0.0)
+    np.testing.assert_allclose(rp.numpy()[fl], rho[fl], rtol=0, atol=1e-6)
+    r2, _, _ = fields_from_prediction(pred, fluid, fx, nu, density=True)
+    r2 = r2.double().numpy()
+    assert np.all(r2[~fl] == 1.0)
+    assert abs(r2[fl].sum() - fl.sum()) < 1e-4
+    # the deviation from the fluid mean is the predicted one
+    np.testing.assert_allclose(r2[fl] - 1.0, rho[fl] - rho[fl].mean(), rtol=0, atol=1e-6)
