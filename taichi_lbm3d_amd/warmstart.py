@@ -1,6 +1,6 @@
 """Learned warm start, end to end on one GPU.
 
-    python taichi_lbm3d_amd/warmstart.py [--train-samples 64] [--iters 4000] [--batch 2]
+    python taichi_lbm3d_amd/warmstart.py [--train-samples 64] [--iters 32000] [--batch 2]
                                           [--crop 128] [--eval-samples 32] [--eval-size 192]
                                           [--save-model PATH] [--out PATH]
 
@@ -170,7 +170,7 @@ def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--train-samples", type=int, default=64)
     ap.add_argument("--train-seed", type=int, default=1000)
-    ap.add_argument("--iters", type=int, default=16000)
+    ap.add_argument("--iters", type=int, default=32000)
     ap.add_argument("--batch", type=int, default=2)
     ap.add_argument("--crop", type=int, default=128)
     ap.add_argument("--eval-samples", type=int, default=32)
