@@ -970,22 +970,21 @@ __device__ __forceinline__ void tile_place(const Geo &gm, const uint4 *win, uint
   jrel = (int)(pos / W);
 }
 
+// One tile (kBlock consecutive fluid cells from lb * kBlock) of a time step.
+// win: the workgroup's LDS row window (odd steps).  Every thread of the
+// workgroup must call it (the window is staged behind a barrier).
 template <typename T, bool ODD, bool FORCED>
-__global__ void __launch_bounds__(kBlock, LBM3D_STEP_MINB)
-    k_step(const KP kp, T *__restrict__ g, const int32_t *__restrict__ cells, const uint4 *__restrict__ rows,
-           const uint32_t *__restrict__ face_bits, typename St<T>::R *__restrict__ face_u, int first) {
+__device__ __forceinline__ void step_tile(const KP &kp, T *__restrict__ g, const int32_t *__restrict__ cells,
+                                          const uint4 *__restrict__ rows, const uint32_t *__restrict__ face_bits,
+                                          typename St<T>::R *__restrict__ face_u, int first, uint32_t seed,
+                                          uint32_t lb, uint4 *win) {
   const uint32_t n = (uint32_t)kp.p.n_cells;
-  const uint32_t nb = (n + kBlock - 1) / kBlock;
-  const uint32_t lb = logical_block<ODD ? LBM3D_XCD_CHUNK_ODD : LBM3D_XCD_CHUNK_EVEN,
-                                    !ODD && LBM3D_REVERSE_EVEN>(nb);
-  if (lb >= nb) return;  // padding blocks of the grid
   const uint32_t x0 = lb * kBlock;
   const uint32_t x = x0 + threadIdx.x;
-  const Slots<T> sl(g, kp.S, kp.sr_seed);
+  const Slots<T> sl(g, kp.S, seed);
   const Rows rb(rows, kp.geo.n_row_words);
   RowView rv{&rb, nullptr, 0u, 0, 0, 0u, -1, -1, 0};
   if constexpr (ODD && LBM3D_ROW_WINDOW) {
-    __shared__ uint4 win[kWinWords];
     if (kp.p.flags & LBM3D_FLAG_TILES) {
       const uint4 td = rows[kp.geo.n_row_words + lb];  // this block's descriptor
       if (td.z != kNoTile) {
@@ -1004,6 +1003,135 @@ __global__ void __launch_bounds__(kBlock, LBM3D_STEP_MINB)
   }
   if (x >= n) return;
   step_cell<T, ODD, FORCED>(kp, kp.geo, sl, rv, cells, face_bits, face_u, x, first);
+}
+
+template <typename T, bool ODD, bool FORCED>
+__global__ void __launch_bounds__(kBlock, LBM3D_STEP_MINB)
+    k_step(const KP kp, T *__restrict__ g, const int32_t *__restrict__ cells, const uint4 *__restrict__ rows,
+           const uint32_t *__restrict__ face_bits, typename St<T>::R *__restrict__ face_u, int first) {
+  const uint32_t n = (uint32_t)kp.p.n_cells;
+  const uint32_t nb = (n + kBlock - 1) / kBlock;
+  const uint32_t lb = logical_block<ODD ? LBM3D_XCD_CHUNK_ODD : LBM3D_XCD_CHUNK_EVEN,
+                                    !ODD && LBM3D_REVERSE_EVEN>(nb);
+  if (lb >= nb) return;  // padding blocks of the grid
+  __shared__ uint4 win[ODD && LBM3D_ROW_WINDOW ? kWinWords : 1];
+  step_tile<T, ODD, FORCED>(kp, g, cells, rows, face_bits, face_u, first, kp.sr_seed, lb, win);
+}
+
+// ---------------------------------------------------------------------------
+// Step pairs in one pass (lbm3d_run_pairs).  The even update of a tile only
+// touches the tile's own slots; the odd update of a tile touches slots of cells
+// in its planes and the two next to them.  The tiles are cut into 8 slabs of
+// consecutive tiles, one per XCD; each XCD's items come in a fixed order: the
+// even update of every tile, and the odd update of each interior tile right
+// after the even update of the last tile in its dependency planes (interior:
+// all tiles of those planes lie in the slab).  Each workgroup takes one item
+// by ticket from its own XCD's counter (the XCD is read from the hardware
+// register), so producer and consumer share the XCD's L2 and the odd update
+// finds the even update's lines there or in the last-level cache.  Tickets go
+// out in item order and in the order workgroups start, and an odd item waits
+// only for even items with earlier tickets, held by workgroups that already
+// run or have finished: the waits end.  They are bounded all the same: a wait that runs out sets an error
+// bit.  Odd updates of the other tiles follow in a second launch
+// (k_step_list), whose first workgroup also resets the counters.
+// ---------------------------------------------------------------------------
+constexpr int kCtrStride = 32;      // one counter per 128-byte line
+constexpr int kPairSpin = 1 << 14;  // polls of one wait (tens of ms) before it is abandoned
+
+struct PairSched {
+  const int32_t *items;        // per XCD runs of tile ids; ~tile for the odd update
+  const int32_t *need;         // per plane: tiles holding cells of that plane
+  const uint32_t *tile_planes; // per tile: first plane | last plane << 16
+  int32_t *state;              // lbm3d_pair_state_words(nx) words, zero between launches
+  int32_t item_off[9];         // XCD k's items: item_off[k] .. item_off[k+1]-1
+  int32_t nx;
+};
+
+// counters in the state words: tickets of XCD k, then per plane the even
+// updates done, then the error word
+__device__ __forceinline__ int32_t *pair_ticket(const PairSched &ps, uint32_t k) {
+  return ps.state + k * kCtrStride;
+}
+__device__ __forceinline__ int32_t *pair_plane(const PairSched &ps, int q) {
+  return ps.state + (8 + q) * kCtrStride;
+}
+__device__ __forceinline__ int32_t *pair_error(const int32_t *state, int nx) {
+  return const_cast<int32_t *>(state) + (8 + nx) * kCtrStride;
+}
+
+template <typename T, bool FORCED>
+__global__ void __launch_bounds__(kBlock, LBM3D_STEP_MINB)
+    k_step_pair(const KP kp, T *__restrict__ g, const int32_t *__restrict__ cells,
+                const uint4 *__restrict__ rows, const uint32_t *__restrict__ face_bits,
+                typename St<T>::R *__restrict__ face_u, int first, uint32_t odd_seed, const PairSched ps) {
+  __shared__ uint4 win[LBM3D_ROW_WINDOW ? kWinWords : 1];
+  __shared__ int32_t s_item, s_ok;
+  // XCC_ID (hardware register 20), bits 3:0
+  const uint32_t xcd = (uint32_t)__builtin_amdgcn_s_getreg((3 << 11) | (0 << 6) | 20) & 7u;
+  const int32_t i0 = ps.item_off[xcd], i1 = ps.item_off[xcd + 1];
+  if (threadIdx.x == 0)
+    s_item = i0 + __hip_atomic_fetch_add(pair_ticket(ps, xcd), 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  __syncthreads();
+  const int32_t it = s_item;
+  if (it >= i1) return;  // a surplus workgroup of this XCD
+  const int32_t code = ps.items[it];
+  if (code >= 0) {
+    const uint32_t tp = ps.tile_planes[code];
+    step_tile<T, false, FORCED>(kp, g, cells, rows, face_bits, face_u, first, kp.sr_seed, (uint32_t)code, win);
+    __builtin_amdgcn_s_waitcnt(0);  // this wave's stores have reached the L2
+    __syncthreads();
+    if (threadIdx.x == 0)
+      for (int q = (int)(tp & 0xFFFFu); q <= (int)(tp >> 16); ++q)
+        __hip_atomic_fetch_add(pair_plane(ps, q), 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  } else {
+    const uint32_t tile = (uint32_t)~code;
+    const uint32_t tp = ps.tile_planes[tile];
+    if (threadIdx.x < 64) {  // wave 0: lane l waits for plane first - 1 + l
+      const int pf = (int)(tp & 0xFFFFu), pl = (int)(tp >> 16);
+      const int lane = (int)threadIdx.x;
+      int q = pf - 1 + lane;
+      q = q < 0 ? q + ps.nx : (q >= ps.nx ? q - ps.nx : q);
+      const int want = lane <= pl - pf + 2 ? ps.need[q] : 0;
+      bool ok = want == 0;
+      int spins = 0;
+      while (__any(!ok) && spins < kPairSpin) {
+        if (!ok) ok = __hip_atomic_fetch_add(pair_plane(ps, q), 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= want;
+        if (__any(!ok)) __builtin_amdgcn_s_sleep(1);
+        ++spins;
+      }
+      if (threadIdx.x == 0) {
+        s_ok = !__any(!ok);
+        if (!s_ok) __hip_atomic_fetch_or(pair_error(ps.state, ps.nx), 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      }
+    }
+    __syncthreads();
+    asm volatile("buffer_inv sc0" ::: "memory");  // drop L1 lines fetched before those even updates
+    step_tile<T, true, FORCED>(kp, g, cells, rows, face_bits, face_u, 0, odd_seed, tile, win);
+  }
+}
+
+// Odd updates of a list of tiles (the slab-edge tiles of a step pair).  The
+// first workgroup also checks that every XCD took all its items (error bit 2
+// otherwise: fewer workgroups ran there than it had items) and resets the
+// tickets and plane counters for the next pair.
+template <typename T, bool FORCED>
+__global__ void __launch_bounds__(kBlock, LBM3D_STEP_MINB)
+    k_step_list(const KP kp, T *__restrict__ g, const int32_t *__restrict__ cells, const uint4 *__restrict__ rows,
+                const uint32_t *__restrict__ face_bits, typename St<T>::R *__restrict__ face_u, uint32_t seed,
+                const int32_t *__restrict__ list, int32_t n_list, const PairSched ps) {
+  __shared__ uint4 win[LBM3D_ROW_WINDOW ? kWinWords : 1];
+  if (blockIdx.x == 0) {
+    for (int k = (int)threadIdx.x; k < 8; k += kBlock) {
+      int32_t *tk = pair_ticket(ps, (uint32_t)k);
+      const int32_t cnt = ps.item_off[k + 1] - ps.item_off[k];
+      if (*tk < cnt)
+        __hip_atomic_fetch_or(pair_error(ps.state, ps.nx), 2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      *tk = 0;
+    }
+    for (int q = (int)threadIdx.x; q < ps.nx; q += kBlock) *pair_plane(ps, q) = 0;
+  }
+  if ((int)blockIdx.x >= n_list) return;
+  step_tile<T, true, FORCED>(kp, g, cells, rows, face_bits, face_u, 0, seed, (uint32_t)list[blockIdx.x], win);
 }
 
 template <typename T, bool ODD>
@@ -1558,6 +1686,57 @@ int launch_step(hipStream_t s, const KP &kp, void *g, const int32_t *cells, cons
   return check_launch("lbm3d_step");
 }
 
+PairSched pair_sched(const lbm3d_pair_plan &pl) {
+  PairSched ps;
+  ps.items = pl.items;
+  ps.need = pl.need;
+  ps.tile_planes = pl.tile_planes;
+  ps.state = pl.state;
+  for (int k = 0; k < 9; ++k) ps.item_off[k] = pl.item_off[k];
+  ps.nx = pl.nx;
+  return ps;
+}
+
+template <typename T, bool FORCED>
+int launch_pair(hipStream_t s, KP kp, void *g, const int32_t *cells, const uint4 *rows, const uint32_t *face_bits,
+                void *face_u, const lbm3d_pair_plan &pl, int64_t t) {
+  using R = typename St<T>::R;
+  const PairSched ps = pair_sched(pl);
+  kp.sr_seed = sr_seed(kp.p, t);
+  const uint32_t odd_seed = sr_seed(kp.p, t + 1);
+  // one workgroup per item: workgroups go to the XCDs in turn, so 8 times the
+  // largest slab's item count gives every XCD enough; the rest exit at once
+  int32_t most = 0;
+  for (int k = 0; k < 8; ++k) most = std::max(most, pl.item_off[k + 1] - pl.item_off[k]);
+  const unsigned grid = 8u * (unsigned)(most + 8);
+  if (pl.item_off[8] > 0)
+    k_step_pair<T, FORCED><<<grid, kBlock, 0, s>>>(kp, (T *)g, cells, rows, face_bits, (R *)face_u, t == 0,
+                                                   odd_seed, ps);
+  const unsigned lg = pl.n_list > 0 ? (unsigned)pl.n_list : 1u;
+  k_step_list<T, FORCED><<<lg, kBlock, 0, s>>>(kp, (T *)g, cells, rows, face_bits, (R *)face_u, odd_seed, pl.list,
+                                               pl.n_list, ps);
+  return check_launch("lbm3d_run_pairs");
+}
+
+template <typename T>
+int run_pairs(hipStream_t s, const KP &kp, void *g, const int32_t *cells, const uint4 *rows,
+              const uint32_t *face_bits, void *face_u, const lbm3d_pair_plan &pl, int64_t t, int32_t left) {
+  int rc = LBM3D_OK;
+  if (kp.p.n_cells == 0) return rc;
+  if ((t & 1) && left > 0) {
+    if ((rc = launch_step<T>(s, kp, g, cells, rows, face_bits, face_u, t))) return rc;
+    ++t;
+    --left;
+  }
+  for (; left >= 2; left -= 2, t += 2) {
+    rc = kp.p.force_on ? launch_pair<T, true>(s, kp, g, cells, rows, face_bits, face_u, pl, t)
+                       : launch_pair<T, false>(s, kp, g, cells, rows, face_bits, face_u, pl, t);
+    if (rc) return rc;
+  }
+  if (left) rc = launch_step<T>(s, kp, g, cells, rows, face_bits, face_u, t);
+  return rc;
+}
+
 template <typename T>
 int launch_macros(hipStream_t s, const KP &kp, const void *g, const int32_t *cells, const uint4 *rows,
                   const uint32_t *face_bits, const void *face_u, int64_t t, void *rho, void *v) {
@@ -1901,6 +2080,45 @@ int lbm3d_run(void *stream, const lbm3d_params *p, int dtype, void *g, const int
     }
     if (rc) return rc;
   }
+  return LBM3D_OK;
+}
+
+int64_t lbm3d_pair_state_words(int32_t nx) { return nx < 1 ? 0 : (int64_t)(8 + nx + 1) * kCtrStride; }
+
+int lbm3d_run_pairs(void *stream, const lbm3d_params *p, int dtype, void *g, const int32_t *cells,
+                    const lbm3d_row_word *rows, const uint32_t *face_bits, void *face_u,
+                    const lbm3d_pair_plan *plan, int64_t t0, int32_t nsteps) {
+  int rc = solver_args(p, dtype, g, cells, rows, face_bits);
+  if (rc) return rc;
+  if (t0 < 0 || nsteps < 0) return fail(LBM3D_EINVAL, "%s", "negative step index or count");
+  if (!plan) return fail(LBM3D_EINVAL, "%s", "plan is NULL");
+  const lbm3d_pair_plan &pl = *plan;
+  if (pl.nx != p->nx || !pl.state || !pl.need || !pl.tile_planes)
+    return fail(LBM3D_EINVAL, "%s", "plan does not match the grid, or a plan array is NULL");
+  if (pl.item_off[0] != 0 || pl.n_list < 0 || (pl.n_list > 0 && !pl.list) || (pl.item_off[8] > 0 && !pl.items))
+    return fail(LBM3D_EINVAL, "%s", "bad plan item ranges");
+  for (int k = 0; k < 8; ++k)
+    if (pl.item_off[k + 1] < pl.item_off[k]) return fail(LBM3D_EINVAL, "%s", "plan item offsets decrease");
+  const KP kp = make_kp(*p);
+  hipStream_t s = (hipStream_t)stream;
+  const uint4 *r = (const uint4 *)rows;
+  switch (dtype) {
+    case LBM3D_F32: return run_pairs<float>(s, kp, g, cells, r, face_bits, face_u, pl, t0, nsteps);
+    case LBM3D_F16: return run_pairs<__half>(s, kp, g, cells, r, face_bits, face_u, pl, t0, nsteps);
+    case LBM3D_F64: return run_pairs<double>(s, kp, g, cells, r, face_bits, face_u, pl, t0, nsteps);
+    default: return run_pairs<__hip_bfloat16>(s, kp, g, cells, r, face_bits, face_u, pl, t0, nsteps);
+  }
+}
+
+int lbm3d_pair_status(void *stream, const lbm3d_pair_plan *plan, int32_t *bits) {
+  if (!plan || !plan->state || !bits || plan->nx < 1) return fail(LBM3D_EINVAL, "%s", "NULL plan, state or output");
+  hipStream_t s = (hipStream_t)stream;
+  int32_t *w = plan->state + (int64_t)(8 + plan->nx) * kCtrStride;
+  int32_t v = 0;
+  if (hipMemcpyAsync(&v, w, sizeof(v), hipMemcpyDeviceToHost, s) != hipSuccess ||
+      hipMemsetAsync(w, 0, sizeof(v), s) != hipSuccess || hipStreamSynchronize(s) != hipSuccess)
+    return fail(LBM3D_EHIP, "%s", "lbm3d_pair_status: copy failed");
+  *bits = v;
   return LBM3D_OK;
 }
 
