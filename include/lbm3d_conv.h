@@ -68,6 +68,15 @@ int lbm3d_conv3d_wgrad(void *stream, const uint16_t *x, const uint16_t *dy, floa
                        int32_t h, int32_t wd, int32_t cin, int32_t cout, int32_t ks, void *workspace,
                        size_t workspace_bytes);
 
+/* Backward of a fused bias + ReLU: dz = g * (y > 0) elementwise (y the
+ * layer's BF16 output; a masked element is g * 0.0 as PyTorch computes it),
+ * or dz = g when y is NULL, and db[c] = sum over voxels of dz[v][c] in FP32,
+ * through per-block partial sums in `workspace` folded in a fixed order.
+ * g, y, dz: [voxels][c] BF16 (dz may be g); c divides 2048, 8..256. */
+size_t lbm3d_relu_bias_grad_workspace_bytes(int64_t voxels, int32_t c);
+int lbm3d_relu_bias_grad(void *stream, const uint16_t *g, const uint16_t *y, uint16_t *dz, float *db,
+                         int64_t voxels, int32_t c, void *workspace, size_t workspace_bytes);
+
 /* The surrogate's distance input: dist[v] = Euclidean distance from voxel v to
  * the nearest solid voxel (solid[v] != 0) of its volume, with every volume
  * periodic, exact up to `margin` voxels and capped there (also when a volume

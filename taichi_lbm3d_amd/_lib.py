@@ -147,6 +147,8 @@ SIGNATURES = {
     "lbm3d_conv3d_wgrad_workspace_bytes": (_sz, [_i32, _i32, _i32, _i32, _i32, _i32, _i32]),
     "lbm3d_conv3d_wgrad": (_int, [_vp, _vp, _vp, _vp, _i32, _i32, _i32, _i32, _i32, _i32, _i32, _vp, _sz]),
     "lbm3d_distance_periodic": (_int, [_vp, _vp, _vp, _vp, _i32, _i32, _i32, _i32, _i32]),
+    "lbm3d_relu_bias_grad_workspace_bytes": (_sz, [_i64, _i32]),
+    "lbm3d_relu_bias_grad": (_int, [_vp, _vp, _vp, _vp, _vp, _i64, _i32, _vp, _sz]),
     # include/lbm3d_io.h
     "lbm3d_io_last_error": (ctypes.c_char_p, []),
     "lbm3d_read_geo_ascii": (_i64, [ctypes.c_char_p, _i32, _i32, _i32, _vp]),

@@ -784,6 +784,72 @@ __global__ void __launch_bounds__(256)
   g[v] = last ? fminf((float)sqrt((double)best), (float)margin) : best;
 }
 
+// bf16 y > 0 (false for -0, negatives and NaN)
+__device__ __forceinline__ bool bf16_positive(uint32_t h) {
+  return !(h & 0x8000u) && h != 0u && !((h & 0x7F80u) == 0x7F80u && (h & 0x7Fu));
+}
+
+__device__ __forceinline__ float bf16_to_f32(uint32_t h) { return __uint_as_float(h << 16); }
+
+// Backward of y = relu(z): dz = g * (y > 0) (a masked element is g * 0.0 in
+// FP32, rounded to bf16: a zero with g's sign, or NaN), stored to dz; with
+// y == nullptr dz = g.  Each block also sums dz per channel over its voxels
+// in FP32: part[block][c].  Data are [voxels][C] bf16, C dividing 2048, so a
+// thread's 8-element chunks (chunk q covers elements 8q .. 8q+7) always hold
+// the same 8 channels when the blocks' chunk ranges start on multiples of C/8.
+__global__ void __launch_bounds__(256)
+    k_relu_bias_grad(const uint16_t *g, const uint16_t *__restrict__ y, uint16_t *dz,  // dz may be g
+                     float *__restrict__ part, int64_t chunks, int C, int64_t per_block) {
+  __shared__ float red[256][9];
+  const int t = (int)threadIdx.x;
+  const int64_t q0 = (int64_t)blockIdx.x * per_block;
+  const int64_t q1 = q0 + per_block < chunks ? q0 + per_block : chunks;
+  float acc[8];
+#pragma unroll
+  for (int e = 0; e < 8; ++e) acc[e] = 0.f;
+  for (int64_t q = q0 + t; q < q1; q += 256) {
+    u32x4 a = ld16(g + q * 8);
+    if (y) {
+      const u32x4 m = ld16(y + q * 8);
+#pragma unroll
+      for (int w = 0; w < 4; ++w) {
+        uint32_t lo = a[w] & 0xFFFFu, hi = a[w] >> 16;
+        if (!bf16_positive(m[w] & 0xFFFFu))
+          lo = __bfloat16_as_ushort(__float2bfloat16(bf16_to_f32(lo) * 0.0f));
+        if (!bf16_positive(m[w] >> 16))
+          hi = __bfloat16_as_ushort(__float2bfloat16(bf16_to_f32(hi) * 0.0f));
+        a[w] = lo | (hi << 16);
+      }
+    }
+    *reinterpret_cast<u32x4 *>(dz + q * 8) = a;
+#pragma unroll
+    for (int w = 0; w < 4; ++w) {
+      acc[2 * w] += bf16_to_f32(a[w] & 0xFFFFu);
+      acc[2 * w + 1] += bf16_to_f32(a[w] >> 16);
+    }
+  }
+#pragma unroll
+  for (int e = 0; e < 8; ++e) red[t][e] = acc[e];
+  __syncthreads();
+  if (t < C) {  // channel t: threads r, r + C/8, ... hold it in slot t % 8 (r = t / 8)
+    const int groups = C / 8;
+    float sum = 0.f;
+    for (int u = t / 8; u < 256; u += groups) sum += red[u][t % 8];
+    part[(int64_t)blockIdx.x * C + t] = sum;
+  }
+}
+
+int relu_bias_plan(int64_t voxels, int C, int64_t &chunks, int64_t &per_block) {
+  chunks = voxels * C / 8;
+  int64_t blocks = (chunks + 256 * 8 - 1) / (256 * 8);  // about 8 chunks per thread
+  if (blocks > 2048) blocks = 2048;
+  if (blocks < 1) blocks = 1;
+  const int64_t unit = C / 8;
+  per_block = (chunks + blocks - 1) / blocks;
+  per_block = (per_block + unit - 1) / unit * unit;
+  return (int)((chunks + per_block - 1) / per_block);
+}
+
 }  // namespace
 
 extern "C" {
@@ -899,6 +965,28 @@ int lbm3d_conv3d_wgrad(void *stream, const uint16_t *x, const uint16_t *dy, floa
   const int64_t total = (int64_t)cout * kpad;
   k_wgrad_fold<<<(unsigned)((total + 255) / 256), 256, 0, s>>>(part, dw, parts * kWgWaves, 1, taps, cin, cout, kpad);
   return ccheck("lbm3d_conv3d_wgrad/fold");
+}
+
+size_t lbm3d_relu_bias_grad_workspace_bytes(int64_t voxels, int32_t c) {
+  if (voxels < 1 || c < 8 || c > 256 || 2048 % c) return 0;
+  int64_t chunks, per;
+  return (size_t)relu_bias_plan(voxels, c, chunks, per) * c * sizeof(float);
+}
+
+int lbm3d_relu_bias_grad(void *stream, const uint16_t *g, const uint16_t *y, uint16_t *dz, float *db, int64_t voxels,
+                         int32_t c, void *workspace, size_t workspace_bytes) {
+  if (voxels < 1) return cfail(LBM3D_EINVAL, "no voxels");
+  if (c < 8 || c > 256 || 2048 % c) return cfail(LBM3D_EINVAL, "channels must divide 2048, 8..256");
+  if (!g || !dz || !db || !workspace) return cfail(LBM3D_EINVAL, "NULL tensor or workspace");
+  if (workspace_bytes < lbm3d_relu_bias_grad_workspace_bytes(voxels, c))
+    return cfail(LBM3D_EINVAL, "relu/bias workspace too small");
+  int64_t chunks, per;
+  const int blocks = relu_bias_plan(voxels, c, chunks, per);
+  hipStream_t s = (hipStream_t)stream;
+  k_relu_bias_grad<<<(unsigned)blocks, 256, 0, s>>>(g, y, dz, (float *)workspace, chunks, c, per);
+  if (int rc = ccheck("lbm3d_relu_bias_grad")) return rc;
+  // the wgrad fold with one tap and one output row: db[k] = sum over blocks of part[block][k]
+  return wgrad_fold(s, (float *)workspace, db, blocks, 1, c, 1, c);
 }
 
 int lbm3d_distance_periodic(void *stream, const int8_t *solid, float *dist, float *work, int32_t n, int32_t d,

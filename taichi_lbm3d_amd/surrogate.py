@@ -92,6 +92,24 @@ def conv_wgrad(x, dy, ks):
     return dw
 
 
+def relu_bias_grad(g, y=None):
+    """(dz, db) for y = relu(z) with z = conv + b: dz = g * (y > 0) (or g
+    without y) in BF16 and db = sum of dz over the voxels in FP32, in one pass
+    (lbm3d_relu_bias_grad).  g, y: (..., C) BF16 channels-last."""
+    torch = _torch()
+    g = g.contiguous()
+    c = g.shape[-1]
+    vox = g.numel() // c
+    lib = _lib.lib()
+    ws_bytes = lib.size("lbm3d_relu_bias_grad_workspace_bytes", vox, c)
+    ws = torch.empty(max(ws_bytes, 4), dtype=torch.uint8, device=g.device)
+    dz = torch.empty_like(g)
+    db = torch.empty(c, dtype=torch.float32, device=g.device)
+    lib.call("lbm3d_relu_bias_grad", _stream(g.device), _vp(g), _vp(y.contiguous()) if y is not None else None,
+             _vp(dz), _vp(db), vox, c, _vp(ws), ws_bytes)
+    return dz, db
+
+
 def _make_conv_function():
     torch = _torch()
 
@@ -112,11 +130,8 @@ def _make_conv_function():
             if ctx.periodic:
                 raise NotImplementedError("periodic convolutions are forward-only (inference)")
             x, wb, y = ctx.saved_tensors
-            g = gy.contiguous().to(torch.bfloat16)
-            if ctx.relu:
-                g = (g * (y > 0)).contiguous()
+            g, db = relu_bias_grad(gy.contiguous().to(torch.bfloat16), y if ctx.relu else None)
             dw = conv_wgrad(x, g, ctx.ks)
-            db = g.float().sum(dim=(0, 1, 2, 3))
             dx = conv_dgrad(g, wb, x.shape[-1], ctx.ks) if ctx.need_dx else None
             return dx, dw, db, None, None, None, None
 

@@ -227,3 +227,36 @@ def test_distance_kernel_matches_oracle(gpu, shape, por, margin):
     empty = np.zeros(shape, np.int8)
     np.testing.assert_array_equal(periodic_distance(empty, margin=margin, device=gpu).cpu().numpy(),
                                   np.full(shape, float(margin), np.float32))
+
+
+@pytest.mark.parametrize("vox,c,relu", [(1, 16, True), (7, 32, True), (1000, 64, False), (2 * 33 ** 3, 16, True),
+                                        (5 * 17 ** 3, 128, True), (300_000, 32, True)])
+def test_relu_bias_grad_matches_torch(gpu, vox, c, relu):
+    """Fused ReLU backward and bias gradient: dz equals PyTorch's g * (y > 0)
+    bit for bit (signed zeros included; NaN where masking a non-finite g gives
+    NaN), db
+    equals the FP32 sum over voxels to rounding, and the result repeats
+    exactly (fixed summation order)."""
+    torch = _torch()
+    from taichi_lbm3d_amd.surrogate import relu_bias_grad
+    gen = torch.Generator().manual_seed(vox + c)
+    g = torch.randn((vox, c), generator=gen).to(torch.bfloat16)
+    y = torch.randn((vox, c), generator=gen).to(torch.bfloat16)
+    if vox >= 1000:  # special values at fixed places
+        g[0, 0], g[1, 1], g[2, 2], g[3, 3] = float("nan"), float("inf"), -float("inf"), -0.0
+        y[0, 0], y[1, 1], y[2, 2], y[4, 4], y[5, 5] = -1.0, 0.0, 1.0, float("nan"), -0.0
+    want = (g * (y > 0)) if relu else g
+    dz, db = relu_bias_grad(g.to(gpu), y.to(gpu) if relu else None)
+    dz, db = dz.cpu(), db.cpu()
+    # NaN where PyTorch has NaN (its payload is platform-defined), every other
+    # element bit for bit, signed zeros included
+    nan = torch.isnan(want)
+    assert torch.equal(torch.isnan(dz), nan)
+    assert torch.equal(dz.view(torch.int16)[~nan], want.view(torch.int16)[~nan])
+    ref = want.double().sum(0)
+    scale = want.double().abs().sum(0)
+    fin = torch.isfinite(ref)
+    assert torch.all((db.double()[fin] - ref[fin]).abs() <= 1e-5 * scale[fin] + 1e-30)
+    assert torch.equal(torch.isnan(db), torch.isnan(ref.float()))
+    dz2, db2 = relu_bias_grad(g.to(gpu), y.to(gpu) if relu else None)
+    assert torch.equal(db2.cpu().view(torch.int32), db.view(torch.int32))
