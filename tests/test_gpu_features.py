@@ -277,3 +277,36 @@ def test_step_pairs_change_nothing(gpu, shape, bc, dtype, splits):
             plan = s._pair_plan[2]
             assert plan["item_off"][8] > 0
     assert torch.equal(out[0], out[1])
+
+
+def test_step_pair_wait_runs_out_and_reports(gpu):
+    """The pair kernel's waits are bounded: with one plane's count of even
+    updates set one higher than can ever arrive (forced from the host), the
+    odd updates that wait for it give up after their poll limit, the launch
+    ends, and lbm3d_pair_status reports error bit 1; a fresh plan runs clean."""
+    import torch
+    from taichi_lbm3d_amd import LB3D_Solver_Single_Phase
+    from taichi_lbm3d_amd.geometry import sphere_pack
+    shape = (96, 48, 40)
+    solid = sphere_pack(shape, porosity=0.4, rmin=2, rmax=5, seed=4, periodic=(False, True, True))
+    s = LB3D_Solver_Single_Phase(*shape, sparse_storage=True)
+    s.use_pairs = True
+    s.solid.from_numpy(solid)
+    s.niu = 0.1
+    s.fx = 1e-5
+    s.init_simulation()
+    s._pairs()
+    plan, arrs, host = s._pair_plan
+    # a plane in the middle of slab 0 that interior odd updates wait for
+    tp = host["tile_planes"]
+    odd = [~int(c) for c in host["items"][host["item_off"][0]:host["item_off"][1]] if c < 0]
+    assert odd
+    q = int(tp[odd[len(odd) // 2]] & 0xFFFF)
+    arrs["need"][q] += 1
+    s.run(2)
+    torch.cuda.synchronize()
+    assert s.pair_status() & 1
+    s._pair_plan = None  # rebuilt from the fluid list
+    s.run(2)
+    torch.cuda.synchronize()
+    assert s.pair_status() == 0
