@@ -186,23 +186,28 @@ def test_abi_rejects_bad_arguments_without_gpu():
 
 
 def test_batch_struct_layouts(tmp_path):
-    """ctypes mirrors of lbm3d_sample, lbm3d_batch_sizes and lbm3d_launch have the C layout."""
-    from taichi_lbm3d_amd._lib import BatchSizes, Launch, Sample
+    """ctypes mirrors of lbm3d_sample, lbm3d_batch_sizes, lbm3d_launch and
+    lbm3d_pair_plan have the C layout."""
+    from taichi_lbm3d_amd._lib import BatchSizes, Launch, PairPlan, Sample
+    structs = ((Sample, "lbm3d_sample"), (BatchSizes, "lbm3d_batch_sizes"), (Launch, "lbm3d_launch"),
+               (PairPlan, "lbm3d_pair_plan"))
     checks = []
-    for cls, cname in ((Sample, "lbm3d_sample"), (BatchSizes, "lbm3d_batch_sizes"), (Launch, "lbm3d_launch")):
+    for cls, cname in structs:
         checks.append(f'printf("{cname}.size %zu\\n", sizeof({cname}));')
         for f in cls._fields_:
             checks.append(f'printf("{cname}.{f[0]} %zu\\n", offsetof({cname}, {f[0]}));')
     src = tmp_path / "batch_layout.c"
     src.write_text('#include <stdio.h>\n#include <stddef.h>\n#include "lbm3d.h"\nint main(void){'
-                   + "".join(checks) + 'printf("align %d\\n", LBM3D_BATCH_ALIGN);return 0;}\n')
+                   + "".join(checks) + 'printf("align %d\\n", LBM3D_BATCH_ALIGN);'
+                   + 'printf("by_block %d\\n", LBM3D_PAIR_BY_BLOCK);return 0;}\n')
     exe = tmp_path / "batch_layout"
     subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), "-o", str(exe), str(src)], check=True)
     got = dict(line.split() for line in subprocess.run([str(exe)], capture_output=True, text=True).stdout.split("\n")
                if line)
     from taichi_lbm3d_amd import _lib
     assert int(got["align"]) == _lib.BATCH_ALIGN
-    for cls, cname in ((Sample, "lbm3d_sample"), (BatchSizes, "lbm3d_batch_sizes"), (Launch, "lbm3d_launch")):
+    assert int(got["by_block"]) == _lib.PAIR_BY_BLOCK
+    for cls, cname in structs:
         assert int(got[cname + ".size"]) == ctypes.sizeof(cls)
         for f in cls._fields_:
             assert int(got[f"{cname}.{f[0]}"]) == getattr(cls, f[0]).offset, (cname, f[0])
