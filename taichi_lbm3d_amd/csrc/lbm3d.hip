@@ -52,6 +52,9 @@ constexpr int kWinWords = 384;       // LDS row window: 3 planes x (J+3) rows x 
 #ifndef LBM3D_STEP_MINB
 #define LBM3D_STEP_MINB 1
 #endif
+#ifndef LBM3D_PAIR_MINB
+#define LBM3D_PAIR_MINB 1  // k_step_pair launch bound; 7 caps it at 72 VGPRs (one spilled) but is no faster
+#endif
 #ifndef LBM3D_REVERSE_EVEN
 // Even steps walk the block range backwards, so they first read the slots the
 // preceding odd step wrote last, while those are still in the 256 MB last-level
@@ -1061,7 +1064,7 @@ __device__ __forceinline__ int32_t *pair_error(const int32_t *state, int nx) {
 }
 
 template <typename T, bool FORCED>
-__global__ void __launch_bounds__(kBlock, LBM3D_STEP_MINB)
+__global__ void __launch_bounds__(kBlock, LBM3D_PAIR_MINB)
     k_step_pair(const KP kp, T *__restrict__ g, const int32_t *__restrict__ cells,
                 const uint4 *__restrict__ rows, const uint32_t *__restrict__ face_bits,
                 typename St<T>::R *__restrict__ face_u, int first, uint32_t odd_seed, const PairSched ps) {
