@@ -207,16 +207,12 @@ int lbm3d_run(void *stream, const lbm3d_params *p, int dtype, void *g, const int
  * other tiles are listed in list[n_list].  need[q] counts the tiles holding
  * cells of plane q.  state holds lbm3d_pair_state_words(nx) int32 words, zero
  * before the first call; the launches leave it zero apart from the error word.
- * Items go to workgroups by per-XCD tickets (the XCD read from the hardware),
- * or with LBM3D_PAIR_BY_BLOCK by workgroup index, which relies on workgroups
- * reaching the XCDs in turn and starting in order on each (error bit 4 when a
- * workgroup finds itself on another XCD). */
-#define LBM3D_PAIR_BY_BLOCK 1 /* workgroup b runs item b / 8 of XCD b % 8 (no tickets) */
+ * Items go to workgroups by per-XCD tickets, the XCD read from the hardware. */
 typedef struct lbm3d_pair_plan {
   int32_t item_off[9];
   int32_t n_list;
   int32_t nx;
-  int32_t flags;                 /* LBM3D_PAIR_BY_BLOCK, or 0: items by per-XCD tickets */
+  int32_t reserved;              /* 0 */
   const int32_t *items;
   const int32_t *list;
   const int32_t *need;
@@ -235,8 +231,7 @@ int lbm3d_run_pairs(void *stream, const lbm3d_params *p, int dtype, void *g, con
                     const lbm3d_pair_plan *plan, int64_t t0, int32_t nsteps);
 
 /* Error bits of the pair launches so far (1: a wait for an even update ran
- * out; 2: an XCD left items untaken; 4: a workgroup ran on another XCD than
- * LBM3D_PAIR_BY_BLOCK assumes); clears them.  Synchronises the stream. */
+ * out; 2: an XCD left items untaken); clears them.  Synchronises the stream. */
 int lbm3d_pair_status(void *stream, const lbm3d_pair_plan *plan, int32_t *bits);
 
 /* Macroscopic fields after t steps, what SP's streaming3 leaves in rho and v

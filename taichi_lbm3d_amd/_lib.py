@@ -19,7 +19,6 @@ ABI_VERSION = 9
 F32, F16, BF16, F64 = 0, 1, 2, 3
 FLAG_STOCHASTIC_ROUNDING = 1
 FLAG_TILES = 2
-PAIR_BY_BLOCK = 1  # LBM3D_PAIR_BY_BLOCK
 # storage name -> storage type code; "...sr" names are the 16-bit types with
 # stochastic rounding (the flag bits of STORAGE_FLAGS set in lbm3d_params.flags)
 DTYPES = {"f32": F32, "f16": F16, "bf16": BF16, "f64": F64, "f16sr": F16, "bf16sr": BF16}
@@ -93,7 +92,7 @@ class Launch(ctypes.Structure):
 class PairPlan(ctypes.Structure):
     """Mirror of struct lbm3d_pair_plan."""
     _fields_ = [("item_off", ctypes.c_int32 * 9), ("n_list", ctypes.c_int32), ("nx", ctypes.c_int32),
-                ("flags", ctypes.c_int32), ("items", ctypes.c_void_p), ("list", ctypes.c_void_p),
+                ("reserved", ctypes.c_int32), ("items", ctypes.c_void_p), ("list", ctypes.c_void_p),
                 ("need", ctypes.c_void_p), ("tile_planes", ctypes.c_void_p), ("state", ctypes.c_void_p)]
 
 

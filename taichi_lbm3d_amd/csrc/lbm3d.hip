@@ -1048,7 +1048,6 @@ struct PairSched {
   int32_t *state;              // lbm3d_pair_state_words(nx) words, zero between launches
   int32_t item_off[9];         // XCD k's items: item_off[k] .. item_off[k+1]-1
   int32_t nx;
-  int32_t by_block;            // 1: workgroup b runs item b / 8 of XCD b % 8, no tickets
 };
 
 // counters in the state words: tickets of XCD k, then per plane the even
@@ -1071,26 +1070,13 @@ __global__ void __launch_bounds__(kBlock, LBM3D_PAIR_MINB)
   __shared__ uint4 win[LBM3D_ROW_WINDOW ? kWinWords : 1];
   __shared__ int32_t s_item, s_ok;
   // XCC_ID (hardware register 20), bits 3:0
-  const uint32_t hw_xcd = (uint32_t)__builtin_amdgcn_s_getreg((3 << 11) | (0 << 6) | 20) & 7u;
-  int32_t it;
-  if (ps.by_block) {
-    // workgroups reach the XCDs in turn and each XCD starts its workgroups in
-    // order, so item b / 8 of XCD b % 8 starts after that XCD's earlier items;
-    // a workgroup on another XCD than b % 8 flags error bit 4 (its L2 would
-    // not hold the even updates it reads)
-    const uint32_t xcd = blockIdx.x & 7u;
-    it = ps.item_off[xcd] + (int32_t)(blockIdx.x >> 3);
-    if (it >= ps.item_off[xcd + 1]) return;
-    if (hw_xcd != xcd && threadIdx.x == 0)
-      __hip_atomic_fetch_or(pair_error(ps.state, ps.nx), 4, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  } else {
-    const int32_t i0 = ps.item_off[hw_xcd], i1 = ps.item_off[hw_xcd + 1];
-    if (threadIdx.x == 0)
-      s_item = i0 + __hip_atomic_fetch_add(pair_ticket(ps, hw_xcd), 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    __syncthreads();
-    it = __builtin_amdgcn_readfirstlane(s_item);  // uniform: tile addresses stay scalar
-    if (it >= i1) return;  // a surplus workgroup of this XCD
-  }
+  const uint32_t xcd = (uint32_t)__builtin_amdgcn_s_getreg((3 << 11) | (0 << 6) | 20) & 7u;
+  const int32_t i0 = ps.item_off[xcd], i1 = ps.item_off[xcd + 1];
+  if (threadIdx.x == 0)
+    s_item = i0 + __hip_atomic_fetch_add(pair_ticket(ps, xcd), 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  __syncthreads();
+  const int32_t it = __builtin_amdgcn_readfirstlane(s_item);  // uniform: tile addresses stay scalar
+  if (it >= i1) return;  // a surplus workgroup of this XCD
   const int32_t code = __builtin_amdgcn_readfirstlane(ps.items[it]);
   if (code >= 0) {
     const uint32_t tp = ps.tile_planes[code];
@@ -1141,7 +1127,7 @@ __global__ void __launch_bounds__(kBlock, LBM3D_STEP_MINB)
     for (int k = (int)threadIdx.x; k < 8; k += kBlock) {
       int32_t *tk = pair_ticket(ps, (uint32_t)k);
       const int32_t cnt = ps.item_off[k + 1] - ps.item_off[k];
-      if (!ps.by_block && *tk < cnt)
+      if (*tk < cnt)
         __hip_atomic_fetch_or(pair_error(ps.state, ps.nx), 2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       *tk = 0;
     }
@@ -1711,7 +1697,6 @@ PairSched pair_sched(const lbm3d_pair_plan &pl) {
   ps.state = pl.state;
   for (int k = 0; k < 9; ++k) ps.item_off[k] = pl.item_off[k];
   ps.nx = pl.nx;
-  ps.by_block = (pl.flags & LBM3D_PAIR_BY_BLOCK) != 0;
   return ps;
 }
 
@@ -1723,11 +1708,11 @@ int launch_pair(hipStream_t s, KP kp, void *g, const int32_t *cells, const uint4
   kp.sr_seed = sr_seed(kp.p, t);
   const uint32_t odd_seed = sr_seed(kp.p, t + 1);
   // one workgroup per item: workgroups go to the XCDs in turn, so 8 times the
-  // largest slab's item count gives every XCD enough (with tickets, 8 more
-  // each for slack); the rest exit at once
+  // largest slab's item count, plus 8 each for slack, gives every XCD enough;
+  // the rest exit at once
   int32_t most = 0;
   for (int k = 0; k < 8; ++k) most = std::max(most, pl.item_off[k + 1] - pl.item_off[k]);
-  const unsigned grid = 8u * (unsigned)(pl.flags & LBM3D_PAIR_BY_BLOCK ? most : most + 8);
+  const unsigned grid = 8u * (unsigned)(most + 8);
   if (pl.item_off[8] > 0)
     k_step_pair<T, FORCED><<<grid, kBlock, 0, s>>>(kp, (T *)g, cells, rows, face_bits, (R *)face_u, t == 0,
                                                    odd_seed, ps);

@@ -161,7 +161,6 @@ class LB3D_Solver_Single_Phase:
         # the even updates of its neighbour planes on the same XCD; identical results
         self.use_pairs = False
         self.pair_lag = 128  # odd updates follow their last dependency by this many more tiles
-        self.pair_by_block = True  # items by workgroup index (b / 8 on XCD b % 8), not per-XCD tickets
         self._initialised = False
         self._compiled = False  # parameters frozen by the first step() (SP kernels compile then)
         self._t = 0
@@ -454,7 +453,6 @@ class LB3D_Solver_Single_Phase:
             for k in range(9):
                 pp.item_off[k] = int(plan["item_off"][k])
             pp.n_list, pp.nx = len(plan["list"]), self.nx
-            pp.flags = _lib.PAIR_BY_BLOCK if self.pair_by_block else 0
             for k in ("items", "list", "need", "tile_planes", "state"):
                 setattr(pp, k, arrs[k].data_ptr() if arrs[k].numel() else None)
             self._pair_plan = (pp, arrs, plan)

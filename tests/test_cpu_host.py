@@ -198,15 +198,13 @@ def test_batch_struct_layouts(tmp_path):
             checks.append(f'printf("{cname}.{f[0]} %zu\\n", offsetof({cname}, {f[0]}));')
     src = tmp_path / "batch_layout.c"
     src.write_text('#include <stdio.h>\n#include <stddef.h>\n#include "lbm3d.h"\nint main(void){'
-                   + "".join(checks) + 'printf("align %d\\n", LBM3D_BATCH_ALIGN);'
-                   + 'printf("by_block %d\\n", LBM3D_PAIR_BY_BLOCK);return 0;}\n')
+                   + "".join(checks) + 'printf("align %d\\n", LBM3D_BATCH_ALIGN);return 0;}\n')
     exe = tmp_path / "batch_layout"
     subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), "-o", str(exe), str(src)], check=True)
     got = dict(line.split() for line in subprocess.run([str(exe)], capture_output=True, text=True).stdout.split("\n")
                if line)
     from taichi_lbm3d_amd import _lib
     assert int(got["align"]) == _lib.BATCH_ALIGN
-    assert int(got["by_block"]) == _lib.PAIR_BY_BLOCK
     for cls, cname in structs:
         assert int(got[cname + ".size"]) == ctypes.sizeof(cls)
         for f in cls._fields_:

@@ -236,16 +236,15 @@ def test_tile_descriptors_change_nothing(gpu, shape, bc):
     assert torch.equal(out[0], out[1])
 
 
-@pytest.mark.parametrize("shape,bc,dtype,splits,by_block,lag", [
-    ((96, 48, 40), "pressure", "f32", (21,), True, 128),
-    ((96, 48, 40), "pressure", "f32", (21,), False, 0),
-    ((80, 33, 70), "force", "f32", (1, 8, 13), True, 0),
-    ((80, 33, 70), "force", "f32", (1, 8, 13), False, 128),
-    ((72, 40, 36), "velocity", "f32", (10, 10), True, 128),
-    ((96, 32, 64), "force", "f16sr", (6, 7), True, 128),
-    ((6, 20, 24), "force", "f32", (9,), True, 128),
+@pytest.mark.parametrize("shape,bc,dtype,splits,lag", [
+    ((96, 48, 40), "pressure", "f32", (21,), 128),
+    ((96, 48, 40), "pressure", "f32", (21,), 0),
+    ((80, 33, 70), "force", "f32", (1, 8, 13), 128),
+    ((72, 40, 36), "velocity", "f32", (10, 10), 128),
+    ((96, 32, 64), "force", "f16sr", (6, 7), 128),
+    ((6, 20, 24), "force", "f32", (9,), 128),
 ])
-def test_step_pairs_change_nothing(gpu, shape, bc, dtype, splits, by_block, lag):
+def test_step_pairs_change_nothing(gpu, shape, bc, dtype, splits, lag):
     """lbm3d_run_pairs (even updates and the odd updates of interior tiles in
     one launch, slab-edge odd updates in a second) gives the same populations,
     bit for bit, as one launch per step: from even and odd starting steps, with
@@ -259,7 +258,7 @@ def test_step_pairs_change_nothing(gpu, shape, bc, dtype, splits, by_block, lag)
     for pairs in (False, True):
         s = LB3D_Solver_Single_Phase(*shape, sparse_storage=True, dtype=dtype)
         s.use_pairs = pairs
-        s.pair_by_block, s.pair_lag = by_block, lag
+        s.pair_lag = lag
         s.solid.from_numpy(solid)
         s.niu = 0.1
         if bc == "pressure":
@@ -294,7 +293,6 @@ def test_step_pair_wait_runs_out_and_reports(gpu):
     solid = sphere_pack(shape, porosity=0.4, rmin=2, rmax=5, seed=4, periodic=(False, True, True))
     s = LB3D_Solver_Single_Phase(*shape, sparse_storage=True)
     s.use_pairs = True
-    s.pair_by_block = False  # per-XCD tickets
     s.solid.from_numpy(solid)
     s.niu = 0.1
     s.fx = 1e-5
