@@ -1068,7 +1068,7 @@ __global__ void __launch_bounds__(kBlock, LBM3D_PAIR_MINB)
                 const uint4 *__restrict__ rows, const uint32_t *__restrict__ face_bits,
                 typename St<T>::R *__restrict__ face_u, int first, uint32_t odd_seed, const PairSched ps) {
   __shared__ uint4 win[LBM3D_ROW_WINDOW ? kWinWords : 1];
-  __shared__ int32_t s_item, s_ok;
+  __shared__ int32_t s_item;
   // XCC_ID (hardware register 20), bits 3:0
   const uint32_t xcd = (uint32_t)__builtin_amdgcn_s_getreg((3 << 11) | (0 << 6) | 20) & 7u;
   const int32_t i0 = ps.item_off[xcd], i1 = ps.item_off[xcd + 1];
@@ -1102,10 +1102,9 @@ __global__ void __launch_bounds__(kBlock, LBM3D_PAIR_MINB)
         if (__any(!ok)) __builtin_amdgcn_s_sleep(1);
         ++spins;
       }
-      if (threadIdx.x == 0) {
-        s_ok = !__any(!ok);
-        if (!s_ok) __hip_atomic_fetch_or(pair_error(ps.state, ps.nx), 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      }
+      const bool all_ok = !__any(!ok);  // the whole wave votes, before lane 0 reports
+      if (threadIdx.x == 0 && !all_ok)
+        __hip_atomic_fetch_or(pair_error(ps.state, ps.nx), 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
     __syncthreads();
     asm volatile("buffer_inv sc0" ::: "memory");  // drop L1 lines fetched before those even updates

@@ -299,7 +299,8 @@ def test_step_pair_wait_runs_out_and_reports(gpu):
     s.init_simulation()
     s._pairs()
     plan, arrs, host = s._pair_plan
-    # a plane in the middle of slab 0 that interior odd updates wait for
+    # a plane in the middle of slab 0 that interior odd updates wait for; each
+    # waiting tile sees it on its lane for plane first - 1, first or first + 1
     tp = host["tile_planes"]
     odd = [~int(c) for c in host["items"][host["item_off"][0]:host["item_off"][1]] if c < 0]
     assert odd
