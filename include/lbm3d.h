@@ -58,7 +58,12 @@ extern "C" {
  * at rows + lbm3d_row_words(p), by the lbm3d_tile_words(p) tile descriptors
  * that lbm3d_build_tiles wrote for the same cell list.  Odd steps then take
  * each block's rows from its descriptor and each cell's place from the row
- * words instead of reading the cell list first.  Results are identical. */
+ * words instead of reading the cell list first.  Results are identical.
+ * Batches: the descriptors follow the batch row table (lbm3d_batch_sizes.
+ * row_words), one per 256-cell block of the batch's cell range; sample s's
+ * start at row_words + cell_begin / 256, written by lbm3d_build_tiles with the
+ * sample's own params and cells.  Batch odd steps launched with 256-thread
+ * blocks and no persistent grid use them. */
 #define LBM3D_FLAG_TILES 2
 
 /* error codes */

@@ -37,7 +37,8 @@ class BatchSolver:
     """
 
     def __init__(self, solids: Sequence[np.ndarray], *, niu: float = 0.16667, force=(0.0, 0.0, 0.0),
-                 faces: Optional[dict] = None, convention: str = "reference", dtype: str = "f32", device=None):
+                 faces: Optional[dict] = None, convention: str = "reference", dtype: str = "f32", device=None,
+                 use_tiles: bool = False):
         if not solids:
             raise ValueError("a batch needs at least one sample")
         if convention not in CONVENTIONS:
@@ -61,6 +62,10 @@ class BatchSolver:
         self.n_samples = len(self.solids)
         self._t = 0
         self.launch = None  # (even, odd) lbm3d_launch shapes; None = library defaults
+        # use_tiles: odd steps with 256-thread blocks locate their cells through
+        # tile descriptors (LBM3D_FLAG_TILES); identical results, but slower in
+        # a batch (the descriptor load waits for the sample lookup), so off
+        self.use_tiles = bool(use_tiles)
         self._graph = None
         self._build()
 
@@ -131,7 +136,8 @@ class BatchSolver:
             self.p.n_cells = int(sizes.cells)
             total = int(sizes.cells)
             cells = torch.full((total,), -1, dtype=torch.int32, device=dev)
-            rows = torch.empty(max(4 * int(sizes.row_words), 4), dtype=torch.int32, device=dev)
+            n_tiles = total // _lib.TILE_CELLS if self.use_tiles else 0
+            rows = torch.empty(max(4 * (int(sizes.row_words) + n_tiles), 4), dtype=torch.int32, device=dev)
             face_bits = torch.zeros(lib.size("lbm3d_face_bit_words", total), dtype=torch.int32, device=dev)
             for s in range(n):
                 sm = self.samples[s]
@@ -144,7 +150,14 @@ class BatchSolver:
                          ctypes_void(rows.data_ptr() + 16 * int(sm.rows_begin)))
                 lib.call("lbm3d_build_face_bits", st, _ref(q), _ptr(solids_d[s]),
                          ctypes_void(cells.data_ptr() + 4 * cb), ctypes_void(face_bits.data_ptr() + 4 * (cb // 16)))
+                if n_tiles:
+                    lib.call("lbm3d_build_tiles", st, _ref(q), ctypes_void(cells.data_ptr() + 4 * cb),
+                             ctypes_void(rows.data_ptr() + 16 * (int(sizes.row_words) + cb // _lib.TILE_CELLS)))
                 del d2c
+            if n_tiles:
+                self.p.flags |= _lib.FLAG_TILES
+            else:
+                self.p.flags &= ~_lib.FLAG_TILES
             host_desc = np.zeros(int(sizes.desc_bytes), np.uint8)
             lib.call("lbm3d_batch_desc", _ref(self.p), self.samples, n, ctypes_void(host_desc.ctypes.data),
                      int(sizes.desc_bytes))

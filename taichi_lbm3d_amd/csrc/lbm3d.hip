@@ -1172,12 +1172,14 @@ __global__ void __launch_bounds__(kBlock)
 // lies in one sample and reads that sample's descriptor with scalar loads.
 // ---------------------------------------------------------------------------
 constexpr uint32_t kChunk = LBM3D_BATCH_ALIGN;
+static_assert(kChunk % kBlock == 0, "a block of the batch lies in one sample");
 
 struct SampleDesc {
   Geo geo;
   uint32_t cell_begin, cell_end;    // the sample's fluid cells: compact ids [cell_begin, cell_end)
   uint32_t chunk_begin, chunk_end;  // its kChunk-cell chunks
   uint64_t rows_begin;              // its first row word
+  uint64_t tiles_begin;             // row-buffer index of its first block's tile descriptor (LBM3D_FLAG_TILES)
 };
 
 struct Batch {
@@ -1269,9 +1271,24 @@ __global__ void __launch_bounds__(B)
   RowView rv{&rb, nullptr, 0u, 0, 0, 0u, -1, -1, 0};
   if constexpr (ODD && LBM3D_ROW_WINDOW) {
     __shared__ uint4 win[kWinWords];
-    const int32_t own = cells[x < d.cell_end ? x : d.cell_end - 1];
-    rv = row_window<B>(d.geo, cells, rows + d.rows_begin, rb, x0, d.cell_end, win);
-    rv.cell = own;
+    bool placed = false;
+    if constexpr (B == kBlock) {
+      if (kp.p.flags & LBM3D_FLAG_TILES) {  // as k_step: the window from this block's descriptor
+        const uint4 td = rows[d.tiles_begin + (x0 - d.cell_begin) / kBlock];
+        if (td.z != kNoTile) {
+          rv = tile_window(d.geo, rows + d.rows_begin, rb, td, win);
+          tile_place(d.geo, win, rv.JW, td.z, x, rv.jrel, rv.k);
+        } else {
+          rv.cell = cells[x < d.cell_end ? x : d.cell_end - 1];  // table path
+        }
+        placed = true;
+      }
+    }
+    if (!placed) {
+      const int32_t own = cells[x < d.cell_end ? x : d.cell_end - 1];
+      rv = row_window<B>(d.geo, cells, rows + d.rows_begin, rb, x0, d.cell_end, win);
+      rv.cell = own;
+    }
   }
   if (x < d.cell_end) step_cell<T, ODD, FORCED>(kp, d.geo, sl, rv, cells, face_bits, face_u, x, first);
 }
@@ -2231,7 +2248,7 @@ int lbm3d_batch_desc(const lbm3d_params *p, const lbm3d_sample *samples, int32_t
   memset(host_desc, 0, desc_table_bytes(n_samples));
   SampleDesc *d = (SampleDesc *)host_desc;
   int32_t *map = (int32_t *)((char *)host_desc + desc_table_bytes(n_samples));
-  int64_t expect = 0;
+  int64_t expect = 0, total_rows = 0;
   for (int32_t s = 0; s < n_samples; ++s) {
     const lbm3d_sample &sm = samples[s];
     if (sm.cell_begin != expect) return fail(LBM3D_EINVAL, "%s", "samples not placed by lbm3d_batch_layout");
@@ -2259,7 +2276,10 @@ int lbm3d_batch_desc(const lbm3d_params *p, const lbm3d_sample *samples, int32_t
     d[s].rows_begin = (uint64_t)sm.rows_begin;
     for (int64_t c = c0; c < c1; ++c) map[c] = s;
     expect = c1 * kChunk;
+    total_rows = sm.rows_begin + lbm3d_row_words(&q);
   }
+  // tile descriptors follow the whole row table, one per kBlock cells of the batch
+  for (int32_t s = 0; s < n_samples; ++s) d[s].tiles_begin = (uint64_t)total_rows + d[s].cell_begin / kBlock;
   return LBM3D_OK;
 }
 
