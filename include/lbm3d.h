@@ -39,7 +39,7 @@
 extern "C" {
 #endif
 
-#define LBM3D_ABI_VERSION 9
+#define LBM3D_ABI_VERSION 10
 
 /* storage types for populations; F32, F16 and BF16 compute in FP32, F64 in FP64 */
 #define LBM3D_F32 0
@@ -50,8 +50,13 @@ extern "C" {
 /* lbm3d_params.flags: F16/BF16 population stores of the step kernels round
  * stochastically (unbiased: each stored value equals the exact one in
  * expectation) instead of to nearest, with random bits that depend only on
- * the step, the slot and the element, so a run is reproducible.  It keeps
- * per-step increments far below the 16-bit spacing, e.g. of a weak body
+ * the step t, the writing cell's id within its own grid (a batch sample's ids
+ * count from its cell_begin) and the store's place in the cell's fixed store
+ * order.  A run is reproducible, and a batch sample gets the bits of the same
+ * grid run alone.  The step's seed is computed on the host from t, so a
+ * captured launch replays the bits of the step it was captured at: replaying
+ * a graph of steps with this flag repeats them (BatchSolver refuses it).  It
+ * keeps per-step increments far below the 16-bit spacing, e.g. of a weak body
  * force, from rounding away.  No effect on F32/F64 storage. */
 #define LBM3D_FLAG_STOCHASTIC_ROUNDING 1
 /* lbm3d_params.flags: the row table given to the step functions is followed,
@@ -199,45 +204,6 @@ int lbm3d_step(void *stream, const lbm3d_params *p, int dtype, void *g, const in
 int lbm3d_run(void *stream, const lbm3d_params *p, int dtype, void *g, const int32_t *cells,
               const lbm3d_row_word *rows, const uint32_t *face_bits, void *face_u, int64_t t0,
               int32_t nsteps);
-
-/* ---- step pairs in one pass (no reference counterpart: a schedule of the
- * same steps that keeps an even step's output in cache for the odd step) ---- */
-
-/* The tiles of lbm3d_tile_words (blocks of 256 consecutive fluid cells) are
- * cut into 8 slabs of consecutive tiles, one per XCD.  XCD k runs, in this
- * order, the items items[item_off[k] .. item_off[k+1]-1]: t for the even
- * update of tile t, ~t for the odd update of tile t.  An odd item may only
- * follow the even items of every tile holding cells of the planes next to and
- * inside its own (tile_planes), all in the same slab; the odd updates of the
- * other tiles are listed in list[n_list].  need[q] counts the tiles holding
- * cells of plane q.  state holds lbm3d_pair_state_words(nx) int32 words, zero
- * before the first call; the launches leave it zero apart from the error word.
- * Items go to workgroups by per-XCD tickets, the XCD read from the hardware. */
-typedef struct lbm3d_pair_plan {
-  int32_t item_off[9];
-  int32_t n_list;
-  int32_t nx;
-  int32_t reserved;              /* 0 */
-  const int32_t *items;
-  const int32_t *list;
-  const int32_t *need;
-  const uint32_t *tile_planes; /* first plane | last plane << 16 */
-  int32_t *state;
-} lbm3d_pair_plan;
-
-int64_t lbm3d_pair_state_words(int32_t nx);
-
-/* lbm3d_run with each (even, odd) step pair as two launches: the plan's items
- * on every XCD, then the listed odd updates.  A leading odd step and a
- * trailing even step run as in lbm3d_step.  Results are identical to
- * lbm3d_run's. */
-int lbm3d_run_pairs(void *stream, const lbm3d_params *p, int dtype, void *g, const int32_t *cells,
-                    const lbm3d_row_word *rows, const uint32_t *face_bits, void *face_u,
-                    const lbm3d_pair_plan *plan, int64_t t0, int32_t nsteps);
-
-/* Error bits of the pair launches so far (1: a wait for an even update ran
- * out; 2: an XCD left items untaken); clears them.  Synchronises the stream. */
-int lbm3d_pair_status(void *stream, const lbm3d_pair_plan *plan, int32_t *bits);
 
 /* Macroscopic fields after t steps, what SP's streaming3 leaves in rho and v
  * (SP:376-388): rho[n_cells] and v (SoA [3][n_cells]), FP32 (FP64 for

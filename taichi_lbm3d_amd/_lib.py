@@ -15,7 +15,7 @@ LIB_PATH = os.path.join(_HERE, "liblbm3d.so")
 SOURCES = [os.path.join(_HERE, "csrc", name) for name in ("lbm3d.hip", "conv3d.hip", "geo_io.hip")]
 INCLUDE = os.path.join(_ROOT, "include")
 
-ABI_VERSION = 9
+ABI_VERSION = 10
 F32, F16, BF16, F64 = 0, 1, 2, 3
 FLAG_STOCHASTIC_ROUNDING = 1
 FLAG_TILES = 2
@@ -89,15 +89,8 @@ class Launch(ctypes.Structure):
                 ("reserved", ctypes.c_int32)]
 
 
-class PairPlan(ctypes.Structure):
-    """Mirror of struct lbm3d_pair_plan."""
-    _fields_ = [("item_off", ctypes.c_int32 * 9), ("n_list", ctypes.c_int32), ("nx", ctypes.c_int32),
-                ("reserved", ctypes.c_int32), ("items", ctypes.c_void_p), ("list", ctypes.c_void_p),
-                ("need", ctypes.c_void_p), ("tile_planes", ctypes.c_void_p), ("state", ctypes.c_void_p)]
-
-
 BATCH_ALIGN = 512  # LBM3D_BATCH_ALIGN
-TILE_CELLS = 256   # cells per tile (lbm3d_tile_words, lbm3d_pair_plan)
+TILE_CELLS = 256   # cells per tile descriptor (lbm3d_tile_words)
 
 # name -> (restype, argtypes); every symbol include/lbm3d.h declares
 _vp, _i32, _i64, _sz, _int = ctypes.c_void_p, ctypes.c_int32, ctypes.c_int64, ctypes.c_size_t, ctypes.c_int

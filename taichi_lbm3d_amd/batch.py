@@ -252,6 +252,11 @@ class BatchSolver:
                 return
             if graph_steps % 2:
                 raise ValueError("graph_steps must be even (the graph starts on an even step)")
+            if self.p.flags & _lib.FLAG_STOCHASTIC_ROUNDING:
+                # the step's rounding seed is a kernel argument: a replayed graph would
+                # repeat the random bits of the steps it was captured at
+                raise ValueError(f"graph_steps needs round-to-nearest storage; {self.dtype!r} rounds "
+                                 "stochastically with per-step seeds that a replay would repeat")
             # eager steps until t is even and past the first step
             lead = 0
             while (self._t + lead) % 2 or self._t + lead == 0:

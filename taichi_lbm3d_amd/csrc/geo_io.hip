@@ -126,13 +126,18 @@ int64_t lbm3d_read_geo_ascii(const char *path, int32_t nx, int32_t ny, int32_t n
     return iofail(LBM3D_EINVAL, "cannot seek ", path);
   }
   const long size = ftell(f);
+  if (size < 0) {  // not a seekable regular file
+    fclose(f);
+    return iofail(LBM3D_EINVAL, "cannot size ", path);
+  }
   rewind(f);
-  char *buf = (char *)malloc(size > 0 ? (size_t)size : 1);
+  const size_t cap = size > 0 ? (size_t)size : 1;
+  char *buf = (char *)malloc(cap);
   if (!buf) {
     fclose(f);
     return iofail(LBM3D_ESIZE, "out of host memory reading ", path);
   }
-  const size_t got = fread(buf, 1, (size_t)size, f);
+  const size_t got = fread(buf, 1, (size_t)size, f);  // size <= cap
   fclose(f);
   const int64_t total = (int64_t)nx * ny * nz;
   const char *p = buf, *end = buf + got;

@@ -52,9 +52,6 @@ constexpr int kWinWords = 384;       // LDS row window: 3 planes x (J+3) rows x 
 #ifndef LBM3D_STEP_MINB
 #define LBM3D_STEP_MINB 1
 #endif
-#ifndef LBM3D_PAIR_MINB
-#define LBM3D_PAIR_MINB 1  // k_step_pair launch bound; 7 caps it at 72 VGPRs (one spilled) but is no faster
-#endif
 #ifndef LBM3D_REVERSE_EVEN
 // Even steps walk the block range backwards, so they first read the slots the
 // preceding odd step wrote last, while those are still in the 256 MB last-level
@@ -456,13 +453,21 @@ __device__ __forceinline__ uint32_t mix32(uint32_t h) {
   return h;
 }
 
+// xorshift32 step (Marsaglia): the per-store random bits of one cell's stream
+__device__ __forceinline__ uint32_t xorshift32(uint32_t x) {
+  x ^= x << 13;
+  x ^= x >> 17;
+  x ^= x << 5;
+  return x;
+}
+
 // Resource k covers slots 2k-1 and 2k (k = 1..9); resource 0 covers slot 0.
 template <typename T>
 struct Slots {
   using R = typename St<T>::R;
   __amdgpu_buffer_rsrc_t r[10];
   uint32_t Sb;    // slot stride in bytes
-  uint32_t seed;  // nonzero: stochastic rounding of 16-bit stores, random bits from (seed, address)
+  uint32_t seed;  // nonzero: stochastic rounding of 16-bit stores (the step's seed)
   __device__ __forceinline__ Slots(const T *g, uint32_t S, uint32_t sr_seed = 0u) {
     seed = sr_seed;
     Sb = S * (uint32_t)sizeof(T);
@@ -473,25 +478,39 @@ struct Slots {
       r[k] = __builtin_amdgcn_make_buffer_rsrc(base + (size_t)(2 * k - 1) * Sb, (short)0, (int)(2 * Sb),
                                                kBufferFlags);
   }
+  // Start of the random stream of the cell with id `local` in its own grid
+  // (a batch sample's ids count from its first cell): the bits of a store
+  // depend only on the step, that id and the store's place in the cell's
+  // fixed store order, not on where the cell sits in memory.
+  __device__ __forceinline__ uint32_t stream(uint32_t local) const {
+    if constexpr (St<T>::kStochastic) {
+      return seed ? mix32(seed + local * 0x9E3779B9u) | 1u : 0u;  // nonzero: xorshift32 never leaves 0
+    }
+    return 0u;
+  }
   // slot q at byte offset vo within the slot
   __device__ __forceinline__ R ld_slot(int q, uint32_t vo) const {
     return St<T>::ld(r[(q + 1) >> 1], vo, (q > 0 && !(q & 1)) ? Sb : 0u);
   }
-  __device__ __forceinline__ void store(int pair, uint32_t vo, uint32_t so, R v) const {
+  // rs: the cell's stream (advanced by a stochastically rounded store)
+  __device__ __forceinline__ void store(int pair, uint32_t vo, uint32_t so, R v, uint32_t &rs) const {
     if constexpr (St<T>::kStochastic) {
       if (seed) {
-        St<T>::st_sr(r[pair], vo, so, v, mix32(seed + ((vo + so) * 10u + (uint32_t)pair) * 0x9E3779B9u));
+        rs = xorshift32(rs);
+        St<T>::st_sr(r[pair], vo, so, v, rs);
         return;
       }
     }
     St<T>::st(r[pair], vo, so, v);
   }
-  __device__ __forceinline__ void st_slot(int q, uint32_t vo, R v) const {
-    store((q + 1) >> 1, vo, (q > 0 && !(q & 1)) ? Sb : 0u, v);
+  __device__ __forceinline__ void st_slot(int q, uint32_t vo, R v, uint32_t &rs) const {
+    store((q + 1) >> 1, vo, (q > 0 && !(q & 1)) ? Sb : 0u, v, rs);
   }
   // byte offset vo relative to the pair of direction q (vo < 2 Sb)
   __device__ __forceinline__ R ld_pair(int q, uint32_t vo) const { return St<T>::ld(r[(q + 1) >> 1], vo, 0u); }
-  __device__ __forceinline__ void st_pair(int q, uint32_t vo, R v) const { store((q + 1) >> 1, vo, 0u, v); }
+  __device__ __forceinline__ void st_pair(int q, uint32_t vo, R v, uint32_t &rs) const {
+    store((q + 1) >> 1, vo, 0u, v, rs);
+  }
 };
 
 // ---------------------------------------------------------------------------
@@ -795,25 +814,28 @@ __device__ __forceinline__ void collide_cell(const KP &kp, const Geo &gm, const 
 }
 
 // One fluid cell's time step: read (with the previous step's face overwrite),
-// macros, pressure-face state, collision, write back in place.
+// macros, pressure-face state, collision, write back in place.  id0: the
+// first compact id of the cell's grid (a batch sample's cell_begin, else 0).
 template <typename T, bool ODD, bool FORCED>
 __device__ __forceinline__ void step_cell(const KP &kp, const Geo &gm, const Slots<T> &sl, const RowView &rv,
                                           const int32_t *__restrict__ cells, const uint32_t *__restrict__ face_bits,
-                                          typename St<T>::R *__restrict__ face_u, uint32_t x, int first) {
+                                          typename St<T>::R *__restrict__ face_u, uint32_t x, int first,
+                                          uint32_t id0 = 0u) {
   using R = typename St<T>::R;
   R v[kQ];
   uint32_t off[kQ];
   CellCtx cc;
   read_cell<T, ODD>(sl, gm, cells, rv, x, v, off, cc);
   collide_cell<R, FORCED>(kp, gm, cells, face_bits, face_u, x, first, v, cc);
+  uint32_t rs = sl.stream(x - id0);
   if constexpr (!ODD) {
     const uint32_t xb = x * (uint32_t)sizeof(T);
 #pragma unroll
-    for (int q = 0; q < kQ; ++q) sl.st_slot(opp(q), xb, v[q]);
+    for (int q = 0; q < kQ; ++q) sl.st_slot(opp(q), xb, v[q], rs);
   } else {
-    sl.st_slot(0, off[0], v[0]);
+    sl.st_slot(0, off[0], v[0], rs);
 #pragma unroll
-    for (int q = 1; q < kQ; ++q) sl.st_pair(q, off[q], v[q]);
+    for (int q = 1; q < kQ; ++q) sl.st_pair(q, off[q], v[q], rs);
   }
 }
 
@@ -861,8 +883,9 @@ __device__ __forceinline__ void init_cell(const KP &kp, const Geo &gm, const Slo
   }
   const R uu = ux * ux + uy * uy + uz * uz;
   const uint32_t xb = x * (uint32_t)sizeof(T);
+  uint32_t rs = 0u;  // init rounds to nearest (sl has no seed)
 #pragma unroll
-  for (int q = 0; q < kQ; ++q) sl.st_slot(q, xb, feq_shift<R>(q, rho, ux, uy, uz, uu));
+  for (int q = 0; q < kQ; ++q) sl.st_slot(q, xb, feq_shift<R>(q, rho, ux, uy, uz, uu), rs);
   int i, j, k;
   coords(gm, (uint32_t)cells[x], i, j, k);
   const int owner = face_owner(p, gm, i, j, k);
@@ -1021,121 +1044,6 @@ __global__ void __launch_bounds__(kBlock, LBM3D_STEP_MINB)
   step_tile<T, ODD, FORCED>(kp, g, cells, rows, face_bits, face_u, first, kp.sr_seed, lb, win);
 }
 
-// ---------------------------------------------------------------------------
-// Step pairs in one pass (lbm3d_run_pairs).  The even update of a tile only
-// touches the tile's own slots; the odd update of a tile touches slots of cells
-// in its planes and the two next to them.  The tiles are cut into 8 slabs of
-// consecutive tiles, one per XCD; each XCD's items come in a fixed order: the
-// even update of every tile, and the odd update of each interior tile right
-// after the even update of the last tile in its dependency planes (interior:
-// all tiles of those planes lie in the slab).  Each workgroup takes one item
-// by ticket from its own XCD's counter (the XCD is read from the hardware
-// register), so producer and consumer share the XCD's L2 and the odd update
-// finds the even update's lines there or in the last-level cache.  Tickets go
-// out in item order and in the order workgroups start, and an odd item waits
-// only for even items with earlier tickets, held by workgroups that already
-// run or have finished: the waits end.  They are bounded all the same: a wait that runs out sets an error
-// bit.  Odd updates of the other tiles follow in a second launch
-// (k_step_list), whose first workgroup also resets the counters.
-// ---------------------------------------------------------------------------
-constexpr int kCtrStride = 32;      // one counter per 128-byte line
-constexpr int kPairSpin = 1 << 14;  // polls of one wait (tens of ms) before it is abandoned
-
-struct PairSched {
-  const int32_t *items;        // per XCD runs of tile ids; ~tile for the odd update
-  const int32_t *need;         // per plane: tiles holding cells of that plane
-  const uint32_t *tile_planes; // per tile: first plane | last plane << 16
-  int32_t *state;              // lbm3d_pair_state_words(nx) words, zero between launches
-  int32_t item_off[9];         // XCD k's items: item_off[k] .. item_off[k+1]-1
-  int32_t nx;
-};
-
-// counters in the state words: tickets of XCD k, then per plane the even
-// updates done, then the error word
-__device__ __forceinline__ int32_t *pair_ticket(const PairSched &ps, uint32_t k) {
-  return ps.state + k * kCtrStride;
-}
-__device__ __forceinline__ int32_t *pair_plane(const PairSched &ps, int q) {
-  return ps.state + (8 + q) * kCtrStride;
-}
-__device__ __forceinline__ int32_t *pair_error(const int32_t *state, int nx) {
-  return const_cast<int32_t *>(state) + (8 + nx) * kCtrStride;
-}
-
-template <typename T, bool FORCED>
-__global__ void __launch_bounds__(kBlock, LBM3D_PAIR_MINB)
-    k_step_pair(const KP kp, T *__restrict__ g, const int32_t *__restrict__ cells,
-                const uint4 *__restrict__ rows, const uint32_t *__restrict__ face_bits,
-                typename St<T>::R *__restrict__ face_u, int first, uint32_t odd_seed, const PairSched ps) {
-  __shared__ uint4 win[LBM3D_ROW_WINDOW ? kWinWords : 1];
-  __shared__ int32_t s_item;
-  // XCC_ID (hardware register 20), bits 3:0
-  const uint32_t xcd = (uint32_t)__builtin_amdgcn_s_getreg((3 << 11) | (0 << 6) | 20) & 7u;
-  const int32_t i0 = ps.item_off[xcd], i1 = ps.item_off[xcd + 1];
-  if (threadIdx.x == 0)
-    s_item = i0 + __hip_atomic_fetch_add(pair_ticket(ps, xcd), 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  __syncthreads();
-  const int32_t it = __builtin_amdgcn_readfirstlane(s_item);  // uniform: tile addresses stay scalar
-  if (it >= i1) return;  // a surplus workgroup of this XCD
-  const int32_t code = __builtin_amdgcn_readfirstlane(ps.items[it]);
-  if (code >= 0) {
-    const uint32_t tp = ps.tile_planes[code];
-    step_tile<T, false, FORCED>(kp, g, cells, rows, face_bits, face_u, first, kp.sr_seed, (uint32_t)code, win);
-    __builtin_amdgcn_s_waitcnt(0);  // this wave's stores have reached the L2
-    __syncthreads();
-    if (threadIdx.x == 0)
-      for (int q = (int)(tp & 0xFFFFu); q <= (int)(tp >> 16); ++q)
-        __hip_atomic_fetch_add(pair_plane(ps, q), 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  } else {
-    const uint32_t tile = (uint32_t)~code;
-    const uint32_t tp = ps.tile_planes[tile];
-    if (threadIdx.x < 64) {  // wave 0: lane l waits for plane first - 1 + l
-      const int pf = (int)(tp & 0xFFFFu), pl = (int)(tp >> 16);
-      const int lane = (int)threadIdx.x;
-      int q = pf - 1 + lane;
-      q = q < 0 ? q + ps.nx : (q >= ps.nx ? q - ps.nx : q);
-      const int want = lane <= pl - pf + 2 ? ps.need[q] : 0;
-      bool ok = want == 0;
-      int spins = 0;
-      while (__any(!ok) && spins < kPairSpin) {
-        if (!ok) ok = __hip_atomic_fetch_add(pair_plane(ps, q), 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= want;
-        if (__any(!ok)) __builtin_amdgcn_s_sleep(1);
-        ++spins;
-      }
-      const bool all_ok = !__any(!ok);  // the whole wave votes, before lane 0 reports
-      if (threadIdx.x == 0 && !all_ok)
-        __hip_atomic_fetch_or(pair_error(ps.state, ps.nx), 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    }
-    __syncthreads();
-    asm volatile("buffer_inv sc0" ::: "memory");  // drop L1 lines fetched before those even updates
-    step_tile<T, true, FORCED>(kp, g, cells, rows, face_bits, face_u, 0, odd_seed, tile, win);
-  }
-}
-
-// Odd updates of a list of tiles (the slab-edge tiles of a step pair).  The
-// first workgroup also checks that every XCD took all its items (error bit 2
-// otherwise: fewer workgroups ran there than it had items) and resets the
-// tickets and plane counters for the next pair.
-template <typename T, bool FORCED>
-__global__ void __launch_bounds__(kBlock, LBM3D_STEP_MINB)
-    k_step_list(const KP kp, T *__restrict__ g, const int32_t *__restrict__ cells, const uint4 *__restrict__ rows,
-                const uint32_t *__restrict__ face_bits, typename St<T>::R *__restrict__ face_u, uint32_t seed,
-                const int32_t *__restrict__ list, int32_t n_list, const PairSched ps) {
-  __shared__ uint4 win[LBM3D_ROW_WINDOW ? kWinWords : 1];
-  if (blockIdx.x == 0) {
-    for (int k = (int)threadIdx.x; k < 8; k += kBlock) {
-      int32_t *tk = pair_ticket(ps, (uint32_t)k);
-      const int32_t cnt = ps.item_off[k + 1] - ps.item_off[k];
-      if (*tk < cnt)
-        __hip_atomic_fetch_or(pair_error(ps.state, ps.nx), 2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      *tk = 0;
-    }
-    for (int q = (int)threadIdx.x; q < ps.nx; q += kBlock) *pair_plane(ps, q) = 0;
-  }
-  if ((int)blockIdx.x >= n_list) return;
-  step_tile<T, true, FORCED>(kp, g, cells, rows, face_bits, face_u, 0, seed, (uint32_t)list[blockIdx.x], win);
-}
-
 template <typename T, bool ODD>
 __global__ void __launch_bounds__(kBlock)
     k_macros(const KP kp, const T *__restrict__ g, const int32_t *__restrict__ cells,
@@ -1223,7 +1131,7 @@ __global__ void __launch_bounds__(B)
       rv = row_window<B>(d.geo, cells, rows + d.rows_begin, rb, x0, d.cell_end, win);
       rv.cell = own;
     }
-    if (x < d.cell_end) step_cell<T, ODD, FORCED>(kp, d.geo, sl, rv, cells, face_bits, face_u, x, first);
+    if (x < d.cell_end) step_cell<T, ODD, FORCED>(kp, d.geo, sl, rv, cells, face_bits, face_u, x, first, d.cell_begin);
   }
 }
 
@@ -1258,8 +1166,9 @@ __global__ void __launch_bounds__(B)
       if (x >= d.cell_end) return;
       collide_cell<R, FORCED>(kp, d.geo, cells, face_bits, face_u, x, first, v, cc);
       const uint32_t xb = x * (uint32_t)sizeof(T);
+      uint32_t rs = sl.stream(x - d.cell_begin);
 #pragma unroll
-      for (int q = 0; q < kQ; ++q) sl.st_slot(opp(q), xb, v[q]);
+      for (int q = 0; q < kQ; ++q) sl.st_slot(opp(q), xb, v[q], rs);
       return;
     }
   }
@@ -1290,7 +1199,7 @@ __global__ void __launch_bounds__(B)
       rv.cell = own;
     }
   }
-  if (x < d.cell_end) step_cell<T, ODD, FORCED>(kp, d.geo, sl, rv, cells, face_bits, face_u, x, first);
+  if (x < d.cell_end) step_cell<T, ODD, FORCED>(kp, d.geo, sl, rv, cells, face_bits, face_u, x, first, d.cell_begin);
 }
 
 template <typename T, bool ODD>
@@ -1705,58 +1614,6 @@ int launch_step(hipStream_t s, const KP &kp, void *g, const int32_t *cells, cons
   return check_launch("lbm3d_step");
 }
 
-PairSched pair_sched(const lbm3d_pair_plan &pl) {
-  PairSched ps;
-  ps.items = pl.items;
-  ps.need = pl.need;
-  ps.tile_planes = pl.tile_planes;
-  ps.state = pl.state;
-  for (int k = 0; k < 9; ++k) ps.item_off[k] = pl.item_off[k];
-  ps.nx = pl.nx;
-  return ps;
-}
-
-template <typename T, bool FORCED>
-int launch_pair(hipStream_t s, KP kp, void *g, const int32_t *cells, const uint4 *rows, const uint32_t *face_bits,
-                void *face_u, const lbm3d_pair_plan &pl, int64_t t) {
-  using R = typename St<T>::R;
-  const PairSched ps = pair_sched(pl);
-  kp.sr_seed = sr_seed(kp.p, t);
-  const uint32_t odd_seed = sr_seed(kp.p, t + 1);
-  // one workgroup per item: workgroups go to the XCDs in turn, so 8 times the
-  // largest slab's item count, plus 8 each for slack, gives every XCD enough;
-  // the rest exit at once
-  int32_t most = 0;
-  for (int k = 0; k < 8; ++k) most = std::max(most, pl.item_off[k + 1] - pl.item_off[k]);
-  const unsigned grid = 8u * (unsigned)(most + 8);
-  if (pl.item_off[8] > 0)
-    k_step_pair<T, FORCED><<<grid, kBlock, 0, s>>>(kp, (T *)g, cells, rows, face_bits, (R *)face_u, t == 0,
-                                                   odd_seed, ps);
-  const unsigned lg = pl.n_list > 0 ? (unsigned)pl.n_list : 1u;
-  k_step_list<T, FORCED><<<lg, kBlock, 0, s>>>(kp, (T *)g, cells, rows, face_bits, (R *)face_u, odd_seed, pl.list,
-                                               pl.n_list, ps);
-  return check_launch("lbm3d_run_pairs");
-}
-
-template <typename T>
-int run_pairs(hipStream_t s, const KP &kp, void *g, const int32_t *cells, const uint4 *rows,
-              const uint32_t *face_bits, void *face_u, const lbm3d_pair_plan &pl, int64_t t, int32_t left) {
-  int rc = LBM3D_OK;
-  if (kp.p.n_cells == 0) return rc;
-  if ((t & 1) && left > 0) {
-    if ((rc = launch_step<T>(s, kp, g, cells, rows, face_bits, face_u, t))) return rc;
-    ++t;
-    --left;
-  }
-  for (; left >= 2; left -= 2, t += 2) {
-    rc = kp.p.force_on ? launch_pair<T, true>(s, kp, g, cells, rows, face_bits, face_u, pl, t)
-                       : launch_pair<T, false>(s, kp, g, cells, rows, face_bits, face_u, pl, t);
-    if (rc) return rc;
-  }
-  if (left) rc = launch_step<T>(s, kp, g, cells, rows, face_bits, face_u, t);
-  return rc;
-}
-
 template <typename T>
 int launch_macros(hipStream_t s, const KP &kp, const void *g, const int32_t *cells, const uint4 *rows,
                   const uint32_t *face_bits, const void *face_u, int64_t t, void *rho, void *v) {
@@ -2100,45 +1957,6 @@ int lbm3d_run(void *stream, const lbm3d_params *p, int dtype, void *g, const int
     }
     if (rc) return rc;
   }
-  return LBM3D_OK;
-}
-
-int64_t lbm3d_pair_state_words(int32_t nx) { return nx < 1 ? 0 : (int64_t)(8 + nx + 1) * kCtrStride; }
-
-int lbm3d_run_pairs(void *stream, const lbm3d_params *p, int dtype, void *g, const int32_t *cells,
-                    const lbm3d_row_word *rows, const uint32_t *face_bits, void *face_u,
-                    const lbm3d_pair_plan *plan, int64_t t0, int32_t nsteps) {
-  int rc = solver_args(p, dtype, g, cells, rows, face_bits);
-  if (rc) return rc;
-  if (t0 < 0 || nsteps < 0) return fail(LBM3D_EINVAL, "%s", "negative step index or count");
-  if (!plan) return fail(LBM3D_EINVAL, "%s", "plan is NULL");
-  const lbm3d_pair_plan &pl = *plan;
-  if (pl.nx != p->nx || !pl.state || !pl.need || !pl.tile_planes)
-    return fail(LBM3D_EINVAL, "%s", "plan does not match the grid, or a plan array is NULL");
-  if (pl.item_off[0] != 0 || pl.n_list < 0 || (pl.n_list > 0 && !pl.list) || (pl.item_off[8] > 0 && !pl.items))
-    return fail(LBM3D_EINVAL, "%s", "bad plan item ranges");
-  for (int k = 0; k < 8; ++k)
-    if (pl.item_off[k + 1] < pl.item_off[k]) return fail(LBM3D_EINVAL, "%s", "plan item offsets decrease");
-  const KP kp = make_kp(*p);
-  hipStream_t s = (hipStream_t)stream;
-  const uint4 *r = (const uint4 *)rows;
-  switch (dtype) {
-    case LBM3D_F32: return run_pairs<float>(s, kp, g, cells, r, face_bits, face_u, pl, t0, nsteps);
-    case LBM3D_F16: return run_pairs<__half>(s, kp, g, cells, r, face_bits, face_u, pl, t0, nsteps);
-    case LBM3D_F64: return run_pairs<double>(s, kp, g, cells, r, face_bits, face_u, pl, t0, nsteps);
-    default: return run_pairs<__hip_bfloat16>(s, kp, g, cells, r, face_bits, face_u, pl, t0, nsteps);
-  }
-}
-
-int lbm3d_pair_status(void *stream, const lbm3d_pair_plan *plan, int32_t *bits) {
-  if (!plan || !plan->state || !bits || plan->nx < 1) return fail(LBM3D_EINVAL, "%s", "NULL plan, state or output");
-  hipStream_t s = (hipStream_t)stream;
-  int32_t *w = plan->state + (int64_t)(8 + plan->nx) * kCtrStride;
-  int32_t v = 0;
-  if (hipMemcpyAsync(&v, w, sizeof(v), hipMemcpyDeviceToHost, s) != hipSuccess ||
-      hipMemsetAsync(w, 0, sizeof(v), s) != hipSuccess || hipStreamSynchronize(s) != hipSuccess)
-    return fail(LBM3D_EHIP, "%s", "lbm3d_pair_status: copy failed");
-  *bits = v;
   return LBM3D_OK;
 }
 

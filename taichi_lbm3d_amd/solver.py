@@ -157,10 +157,6 @@ class LB3D_Solver_Single_Phase:
         # odd steps locate their cells through tile descriptors (LBM3D_FLAG_TILES);
         # results are identical either way
         self.use_tiles = True
-        # step pairs in one pass (lbm3d_run_pairs): the odd update of a tile follows
-        # the even updates of its neighbour planes on the same XCD; identical results
-        self.use_pairs = False
-        self.pair_lag = 128  # odd updates follow their last dependency by this many more tiles
         self._initialised = False
         self._compiled = False  # parameters frozen by the first step() (SP kernels compile then)
         self._t = 0
@@ -331,7 +327,6 @@ class LB3D_Solver_Single_Phase:
             torch.cuda.current_stream(dev).synchronize()
         del d2c, ws, solid
         self._dev.update(cells=cells, rows=rows, face_bits=face_bits)
-        self._pair_plan = None  # built by the first run() with use_pairs
         self._modes_built = self._bc_modes()
 
     def neighbor_table(self) -> np.ndarray:
@@ -426,46 +421,11 @@ class LB3D_Solver_Single_Phase:
         p = self._params
         d = self._dev
         with torch.cuda.device(self._torch_device()):
-            if self.use_pairs:
-                _lib.lib().call("lbm3d_run_pairs", self._stream(), _ref(p), _lib.DTYPES[self.dtype], _ptr(d["g"]),
-                                _ptr(d["cells"]), _ptr(d["rows"]), _ptr(d["face_bits"]), _ptr(d["face_u"]),
-                                _ref(self._pairs()), self._t, int(nsteps))
-            else:
-                _lib.lib().call("lbm3d_run", self._stream(), _ref(p), _lib.DTYPES[self.dtype], _ptr(d["g"]),
-                                _ptr(d["cells"]), _ptr(d["rows"]), _ptr(d["face_bits"]), _ptr(d["face_u"]),
-                                self._t, int(nsteps))
+            _lib.lib().call("lbm3d_run", self._stream(), _ref(p), _lib.DTYPES[self.dtype], _ptr(d["g"]),
+                            _ptr(d["cells"]), _ptr(d["rows"]), _ptr(d["face_bits"]), _ptr(d["face_u"]),
+                            self._t, int(nsteps))
         self._t += int(nsteps)
         self._macro_cache = None
-
-    def _pairs(self):
-        """The lbm3d_pair_plan of the fluid list (built once per list)."""
-        if self._pair_plan is None:
-            torch = _torch()
-            from .pairs import pair_plan
-            dev = self._torch_device()
-            plan = pair_plan(self._dev["cells"][: self.n_fluid].cpu().numpy(), self.nx, self.ny, self.nz,
-                             lag=self.pair_lag)
-            arrs = {k: torch.from_numpy(plan[k].view(np.int32) if plan[k].dtype == np.uint32 else plan[k]).to(dev)
-                    for k in ("items", "list", "need", "tile_planes")}
-            arrs["state"] = torch.zeros(_lib.lib().size("lbm3d_pair_state_words", self.nx), dtype=torch.int32,
-                                        device=dev)
-            pp = _lib.PairPlan()
-            for k in range(9):
-                pp.item_off[k] = int(plan["item_off"][k])
-            pp.n_list, pp.nx = len(plan["list"]), self.nx
-            for k in ("items", "list", "need", "tile_planes", "state"):
-                setattr(pp, k, arrs[k].data_ptr() if arrs[k].numel() else None)
-            self._pair_plan = (pp, arrs, plan)
-        return self._pair_plan[0]
-
-    def pair_status(self) -> int:
-        """Error bits of the pair launches so far (0: none; see lbm3d_pair_status); clears them."""
-        if not self.use_pairs or self._pair_plan is None:
-            return 0
-        import ctypes
-        bits = ctypes.c_int32()
-        _lib.lib().call("lbm3d_pair_status", self._stream(), _ref(self._pair_plan[0]), _ref(bits))
-        return int(bits.value)
 
     @property
     def steps_done(self) -> int:

@@ -186,11 +186,10 @@ def test_abi_rejects_bad_arguments_without_gpu():
 
 
 def test_batch_struct_layouts(tmp_path):
-    """ctypes mirrors of lbm3d_sample, lbm3d_batch_sizes, lbm3d_launch and
-    lbm3d_pair_plan have the C layout."""
-    from taichi_lbm3d_amd._lib import BatchSizes, Launch, PairPlan, Sample
-    structs = ((Sample, "lbm3d_sample"), (BatchSizes, "lbm3d_batch_sizes"), (Launch, "lbm3d_launch"),
-               (PairPlan, "lbm3d_pair_plan"))
+    """ctypes mirrors of lbm3d_sample, lbm3d_batch_sizes and lbm3d_launch have
+    the C layout."""
+    from taichi_lbm3d_amd._lib import BatchSizes, Launch, Sample
+    structs = ((Sample, "lbm3d_sample"), (BatchSizes, "lbm3d_batch_sizes"), (Launch, "lbm3d_launch"))
     checks = []
     for cls, cname in structs:
         checks.append(f'printf("{cname}.size %zu\\n", sizeof({cname}));')
@@ -458,42 +457,3 @@ def test_prediction_fields_keep_the_cold_start_mass():
     assert abs(r2[fl].sum() - fl.sum()) < 1e-4
     # the deviation from the fluid mean is the predicted one
     np.testing.assert_allclose(r2[fl] - 1.0, rho[fl] - rho[fl].mean(), rtol=0, atol=1e-6)
-
-
-@pytest.mark.parametrize("seed,shape,porosity", [(0, (40, 24, 30), 0.4), (1, (60, 9, 70), 0.25), (2, (64, 8, 8), 0.6)])
-def test_pair_plan_orders_every_odd_update_after_its_even_updates(seed, shape, porosity):
-    """The pair schedule: every tile's even update once, on its own slab;
-    every odd update once, either listed for the second launch or after the
-    even update of every tile holding cells of its planes and the ones next to
-    them (wrapped), all on the same slab; need[] counts each plane's tiles."""
-    from taichi_lbm3d_amd.pairs import pair_plan
-    rng = np.random.default_rng(seed)
-    nx, ny, nz = shape
-    solid = rng.random(shape) < 1.0 - porosity
-    cells = np.flatnonzero(~solid.reshape(-1)).astype(np.int32)
-    tile = 32  # small tiles: many tiles and slabs at test sizes
-    plan = pair_plan(cells, nx, ny, nz, tile=tile)
-    n_tiles = (len(cells) + tile - 1) // tile
-    plane = cells // (ny * nz)
-    tiles_of_plane = [set(np.flatnonzero(plane == q) // tile) for q in range(nx)]
-    assert [len(t) for t in tiles_of_plane] == list(plan["need"])
-    bounds = plan["slab_bounds"]
-    evens, odds = [], list(plan["list"])
-    for s in range(8):
-        run = plan["items"][plan["item_off"][s]:plan["item_off"][s + 1]]
-        done = set()
-        for code in run:
-            if code >= 0:
-                assert bounds[s] <= code < bounds[s + 1]
-                done.add(int(code))
-                evens.append(int(code))
-            else:
-                t = int(~code)
-                odds.append(t)
-                pf, pl = int(plan["tile_planes"][t] & 0xFFFF), int(plan["tile_planes"][t] >> 16)
-                assert (pf, pl) == (plane[t * tile], plane[min(t * tile + tile - 1, len(cells) - 1)])
-                for q in range(pf - 1, pl + 2):
-                    assert tiles_of_plane[q % nx] <= done  # every dependency ran before, on this slab
-    assert sorted(evens) == list(range(n_tiles))
-    assert sorted(odds) == list(range(n_tiles))
-    assert len(plan["list"]) < n_tiles  # some tiles are interior at these sizes

@@ -141,13 +141,30 @@ def test_batch_reduce_matches_single(gpu):
         np.testing.assert_allclose(r[s, 4], rs["max_v"], rtol=1e-15)
 
 
-@pytest.mark.parametrize("dtype", ["f16", "bf16", "f64"])
+@pytest.mark.parametrize("dtype", ["f16", "bf16", "f64", "f16sr", "bf16sr"])
 def test_batch_storage_types_bit_exact(gpu, dtype):
+    """Every storage type, stochastic rounding included: a sample's random bits
+    depend on its own cell ids, not on where the batch put them."""
     b, singles = batch_case("pressure_x", SHAPES, seed=4, dtype=dtype)
     b.run(8)
     for s in singles:
         s.run(8)
     assert_same(b, singles)
+
+
+@pytest.mark.parametrize("dtype", ["f16sr", "bf16sr"])
+def test_batch_stochastic_rounding_on_both_parities_and_shapes(gpu, dtype):
+    """Stochastic rounding under every launch shape and an odd step count:
+    the batch still equals each sample run alone, bit for bit; a graph of such
+    steps (whose captured seeds a replay would repeat) is refused."""
+    b, singles = batch_case("periodic_force", SHAPES + [(20, 20, 20)], seed=6, dtype=dtype)
+    b.set_launch((128, 2, 1), (512, 0, 0))
+    b.run(11)
+    for s in singles:
+        s.run(11)
+    assert_same(b, singles)
+    with pytest.raises(ValueError, match="round-to-nearest"):
+        b.run(20, graph_steps=4)
 
 
 def test_batch_sample_fields_and_warm_start(gpu):

@@ -234,82 +234,3 @@ def test_tile_descriptors_change_nothing(gpu, shape, bc):
         out.append(s._dev["g"].cpu())
         assert (s._params.flags & 2) == (2 if tiles else 0)
     assert torch.equal(out[0], out[1])
-
-
-@pytest.mark.parametrize("shape,bc,dtype,splits,lag", [
-    ((96, 48, 40), "pressure", "f32", (21,), 128),
-    ((96, 48, 40), "pressure", "f32", (21,), 0),
-    ((80, 33, 70), "force", "f32", (1, 8, 13), 128),
-    ((72, 40, 36), "velocity", "f32", (10, 10), 128),
-    ((96, 32, 64), "force", "f16sr", (6, 7), 128),
-    ((6, 20, 24), "force", "f32", (9,), 128),
-])
-def test_step_pairs_change_nothing(gpu, shape, bc, dtype, splits, lag):
-    """lbm3d_run_pairs (even updates and the odd updates of interior tiles in
-    one launch, slab-edge odd updates in a second) gives the same populations,
-    bit for bit, as one launch per step: from even and odd starting steps, with
-    pressure, velocity and body-force drives, 16-bit storage with stochastic
-    rounding, and a grid too thin for any interior tile.  No wait runs out."""
-    import torch
-    from taichi_lbm3d_amd import LB3D_Solver_Single_Phase
-    from taichi_lbm3d_amd.geometry import sphere_pack
-    solid = sphere_pack(shape, porosity=0.4, rmin=2, rmax=5, seed=4, periodic=(False, True, True))
-    out = []
-    for pairs in (False, True):
-        s = LB3D_Solver_Single_Phase(*shape, sparse_storage=True, dtype=dtype)
-        s.use_pairs = pairs
-        s.pair_lag = lag
-        s.solid.from_numpy(solid)
-        s.niu = 0.1
-        if bc == "pressure":
-            s.set_bc_rho_x0(1.001)
-            s.set_bc_rho_x1(1.0)
-        elif bc == "velocity":
-            s.set_bc_vel_x0([0.01, 0.0, 0.0])
-            s.set_bc_rho_x1(1.0)
-        else:
-            s.fx = 1e-5
-        s.init_simulation()
-        for k in splits:
-            s.run(k)
-        torch.cuda.synchronize()
-        out.append(s._dev["g"].cpu())
-        if pairs:
-            assert s.pair_status() == 0
-            plan = s._pair_plan[2]
-            assert plan["item_off"][8] > 0
-    assert torch.equal(out[0], out[1])
-
-
-def test_step_pair_wait_runs_out_and_reports(gpu):
-    """The pair kernel's waits are bounded: with one plane's count of even
-    updates set one higher than can ever arrive (forced from the host), the
-    odd updates that wait for it give up after their poll limit, the launch
-    ends, and lbm3d_pair_status reports error bit 1; a fresh plan runs clean."""
-    import torch
-    from taichi_lbm3d_amd import LB3D_Solver_Single_Phase
-    from taichi_lbm3d_amd.geometry import sphere_pack
-    shape = (96, 48, 40)
-    solid = sphere_pack(shape, porosity=0.4, rmin=2, rmax=5, seed=4, periodic=(False, True, True))
-    s = LB3D_Solver_Single_Phase(*shape, sparse_storage=True)
-    s.use_pairs = True
-    s.solid.from_numpy(solid)
-    s.niu = 0.1
-    s.fx = 1e-5
-    s.init_simulation()
-    s._pairs()
-    plan, arrs, host = s._pair_plan
-    # a plane in the middle of slab 0 that interior odd updates wait for; each
-    # waiting tile sees it on its lane for plane first - 1, first or first + 1
-    tp = host["tile_planes"]
-    odd = [~int(c) for c in host["items"][host["item_off"][0]:host["item_off"][1]] if c < 0]
-    assert odd
-    q = int(tp[odd[len(odd) // 2]] & 0xFFFF)
-    arrs["need"][q] += 1
-    s.run(2)
-    torch.cuda.synchronize()
-    assert s.pair_status() & 1
-    s._pair_plan = None  # rebuilt from the fluid list
-    s.run(2)
-    torch.cuda.synchronize()
-    assert s.pair_status() == 0
