@@ -114,9 +114,6 @@ SIGNATURES = {
     "lbm3d_init": (_int, [_vp, _pp, _int, _vp, _vp, _vp, _vp, _vp]),
     "lbm3d_step": (_int, [_vp, _pp, _int, _vp, _vp, _vp, _vp, _vp, _i64]),
     "lbm3d_run": (_int, [_vp, _pp, _int, _vp, _vp, _vp, _vp, _vp, _i64, _i32]),
-    "lbm3d_pair_state_words": (_i64, [_i32]),
-    "lbm3d_run_pairs": (_int, [_vp, _pp, _int, _vp, _vp, _vp, _vp, _vp, ctypes.POINTER(PairPlan), _i64, _i32]),
-    "lbm3d_pair_status": (_int, [_vp, ctypes.POINTER(PairPlan), ctypes.POINTER(_i32)]),
     "lbm3d_macros": (_int, [_vp, _pp, _int, _vp, _vp, _vp, _vp, _vp, _i64, _vp, _vp]),
     "lbm3d_scatter_dense": (_int, [_vp, _pp, _vp, _vp, _vp, _i64, _i32, _i32, _i32, _vp, _vp]),
     "lbm3d_reduce_workspace_bytes": (_sz, [_i32]),
