@@ -66,6 +66,8 @@ def _lib(real: str):
         "set_solid": fn("set_solid", None, [vp, ctypes.c_void_p]),
         "set_params": fn("set_params", None, [vp, ctypes.c_double, ip, ctypes.c_double, ctypes.c_double,
                                                 ctypes.c_double, ctypes.c_void_p, dp, dp]),
+        "set_compile_params": fn("set_compile_params", None, [vp, ctypes.c_double, ctypes.c_double,
+                                                              ctypes.c_double, ctypes.c_void_p, dp]),
         "init_simulation": fn("init_simulation", None, [vp]),
         "step": fn("step", None, [vp, ip]),
         "get_max_v": fn("get_max_v", ctypes.c_double, [vp]),
@@ -91,8 +93,12 @@ class COracle:
     """The C restatement driven with the reference's attribute surface.
 
     Attributes mirror SP:17-28 (``niu``, ``fx/fy/fz``, ``bc_x_left`` ...,
-    ``rho_bcxl`` ..., ``vx_bcxl`` ...); ``init_simulation()`` freezes them
-    like SP:118-149 does.
+    ``rho_bcxl`` ..., ``vx_bcxl`` ...).  They are frozen in two stages, as
+    Taichi freezes the reference's: ``init_simulation()`` fixes S from niu,
+    force_flag and the face velocities (SP:118-140); the first ``step()``
+    fixes what the kernels capture when they are compiled, fx/fy/fz
+    (SP:217-220) and the face modes and densities (SP:274-365).  Changes
+    after the first step have no effect.
     """
 
     def __init__(self, nx, ny, nz, sparse_storage=False, real="f32", convention="reference", threads=1):
@@ -143,8 +149,15 @@ class COracle:
         self._api["set_params"](self._h, float(self.niu), CONVENTIONS[self.convention], float(self.fx),
                                 float(self.fy), float(self.fz), mode.ctypes.data, _dp(rho), _dp(u))
         self._api["init_simulation"](self._h)
+        self._compiled = False
 
     def step(self, n=1):
+        if not getattr(self, "_compiled", True):
+            mode = np.array(self.bc, np.int32)
+            rho = np.array(self.rho_bc, np.float64)
+            self._api["set_compile_params"](self._h, float(self.fx), float(self.fy), float(self.fz),
+                                            mode.ctypes.data, _dp(rho))
+            self._compiled = True
         self._api["step"](self._h, int(n))
 
     def shape(self):

@@ -183,6 +183,20 @@ void ORC(set_params)(ORC(t) * o, double niu, int convention, double fx, double f
   }
 }
 
+/* What SP's kernels capture when they are first compiled, at the first step():
+ * the body force of cal_local_force (SP:217-220) and the face modes and
+ * densities that Boundary_condition tests with ti.static (SP:274-365).  S,
+ * force_flag and the face velocity lists stay as init_simulation froze them
+ * (SP:119-140). */
+void ORC(set_compile_params)(ORC(t) * o, double fx, double fy, double fz, const int32_t *bc_mode,
+                             const double *rho_bc) {
+  o->fx = (REAL)fx; o->fy = (REAL)fy; o->fz = (REAL)fz;
+  for (int s = 0; s < 6; ++s) {
+    o->bc_mode[s] = bc_mode[s];
+    o->rho_bc[s] = (REAL)rho_bc[s];
+  }
+}
+
 /* static_init (SP:180-197) and the M / inv_M fields (SP:61-110). */
 static void static_init(ORC(t) * o) {
   double inv[Q][Q];

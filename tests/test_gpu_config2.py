@@ -1,0 +1,268 @@
+"""Pressure-driven porous flow (the benchmark's physics), the permeability and
+steady-state helpers, Taichi's parameter freezing and the wavefront step
+order, on the GPU against the FP64 oracle and closed forms."""
+import os
+import time
+
+import numpy as np
+import pytest
+
+from oracle import COracle
+from test_gpu_parity import FACES, rel_l2
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+THREADS = min(16, len(os.sched_getaffinity(0)))
+GUO_A = {"reference": 1.0 / 3.0, "documented": 1.0, "textbook": 3.0}
+
+
+def pack32():
+    g = np.load(os.path.join(ROOT, "tests", "golden", "pack32_compaction.npz"))
+    return np.unpackbits(g["bits"])[: 32 ** 3].reshape(32, 32, 32).astype(np.int8)
+
+
+def pressure_solver(solid, dtype="f32", sparse=True, wave=0):
+    from taichi_lbm3d_amd import LB3D_Solver_Single_Phase
+    s = LB3D_Solver_Single_Phase(*solid.shape, sparse_storage=sparse, dtype=dtype)
+    s.solid.from_numpy(solid)
+    s.niu = 0.1
+    s.set_bc_rho_x0(1.0005)
+    s.set_bc_rho_x1(1.0)
+    s.wave_chunks = wave
+    s.init_simulation()
+    return s
+
+
+def test_porous_pressure_run_matches_oracle(gpu):
+    """The committed 32^3 sphere pack with the benchmark's drive (sparse
+    storage, pressure x faces 1.0005 / 1.0, y and z periodic, niu 0.1) for
+    4,000 steps: FP32 and FP64 storage against the FP64 oracle, and
+    permeability() against the pressure-drive Darcy formula evaluated in FP64
+    on the oracle's fields."""
+    solid = pack32()
+    steps = 4000
+    o = COracle(32, 32, 32, sparse_storage=True, real="f64", threads=THREADS)
+    o.set_solid(solid)
+    o.niu = 0.1
+    o.set_bc_rho("x_left", 1.0005)
+    o.set_bc_rho("x_right", 1.0)
+    o.init_simulation()
+    t0 = time.time()
+    o.step(steps)
+    fl = solid == 0
+    orho, ov = o.rho()[:32, :32, :32], o.v()[:32, :32, :32]
+    # Darcy, pressure drive: k = nu_eff q (nx - 1) / (drho / 3), q the superficial
+    # mean x velocity over the slabs 1 <= i <= nx - 2
+    nu = (0.1 / 3.0 + 0.5 - 0.5) / 3.0
+    q = (ov[..., 0] * fl)[1:31].sum() / (32 * 32 * 30)
+    k_np = nu * q * 31 / (0.0005 / 3.0)
+    for dtype, tol_v, tol_k in (("f32", 1e-5, 1e-5), ("f64", 1e-11, 1e-11)):
+        s = pressure_solver(solid, dtype)
+        s.run(steps)
+        rho, v = s.rho.to_numpy()[:32, :32, :32], s.v.to_numpy()[:32, :32, :32]
+        er, ev = rel_l2(rho[fl], orho[fl]), rel_l2(v[fl], ov[fl])
+        k = s.permeability()
+        print(f"{dtype}: rel-L2 rho {er:.2e} v {ev:.2e}, k {k:.6g} vs {k_np:.6g} "
+              f"({abs(k / k_np - 1):.1e}); oracle {time.time() - t0:.1f}s")
+        assert er < tol_v and ev < tol_v
+        assert abs(k / k_np - 1) < tol_k
+    assert k_np > 0
+
+
+def slit(conv, niu, fx, dtype):
+    """example_poiseuille_flow.py's slit (NX 5, NY 20, NZ 16, solid planes
+    z = 0 and z = NZ-1) with the body force along x instead of y."""
+    from taichi_lbm3d_amd import LB3D_Solver_Single_Phase
+    solid = np.zeros((5, 20, 16), np.int8)
+    solid[:, :, 0] = solid[:, :, -1] = 1
+    s = LB3D_Solver_Single_Phase(5, 20, 16, sparse_storage=False, convention=conv, dtype=dtype)
+    s.solid.from_numpy(solid)
+    s.set_force([fx, 0.0, 0.0])
+    s.set_viscosity(niu)
+    s.init_simulation()
+    return s
+
+
+def slit_k_closed(nz=16):
+    """Superficial permeability of the slit: with halfway bounce-back the walls
+    sit at z = 1/2 and NZ - 3/2 (H = NZ - 2), and the momentum profile
+    j + F_eff/2 = F_eff/(2 nu) (z - 1/2)(NZ - 3/2 - z) is exact on the lattice, so
+    k = nu q / F_eff = sum_z (z - 1/2)(H + 1/2 - z) / (2 NZ) = H (2 H^2 + 1) / (24 NZ)."""
+    h = nz - 2
+    return h * (2 * h * h + 1) / (24 * nz)
+
+
+@pytest.mark.parametrize("dtype,tol", [("f64", 1e-9), ("f32", 1e-5)])
+def test_slit_permeability_textbook(gpu, dtype, tol):
+    """Mass-conserving force scaling: permeability() equals the closed form to
+    round-off once run_to_steady() has converged."""
+    s = slit("textbook", 0.1667, 1e-5, dtype)
+    steps, ok, hist = s.run_to_steady(tol=1e-10 if dtype == "f64" else 1e-7, interval=500, max_steps=40000)
+    k, kc = s.permeability(), slit_k_closed()
+    print(dtype, steps, ok, k / kc - 1)
+    assert ok
+    assert abs(k / kc - 1) < tol
+
+
+def test_slit_permeability_reference_scaling(gpu):
+    """SP's force scaling (a = 1/3, b = 1/9) removes mass at -(8/27) u.F per
+    cell and step, so the density drifts below 1; permeability() measures the
+    momentum-consistent flux (j + F_eff/2) and tracks the closed form within
+    that drift (FP64 storage, niu 3 so the slit settles in ~1,000 steps)."""
+    s = slit("reference", 3.0, 1e-5, "f64")
+    steps, ok, hist = s.run_to_steady(tol=1e-6, interval=500, max_steps=20000)
+    assert ok
+    s.run(20000)
+    rho = s.rho.to_numpy()[:, :, 1:-1]
+    drift = abs(1.0 - rho.mean())
+    k, kc = s.permeability(), slit_k_closed()
+    print("drift", drift, "k/kc - 1", k / kc - 1)
+    assert 1e-7 < drift < 1e-4
+    assert abs(k / kc - 1) < 1.5 * drift + 1e-9
+
+
+def test_run_to_steady_stops_by_its_criterion(gpu):
+    """run_to_steady returns at the first check whose relative change of <vx>
+    against the previous check is below tol, every `interval` steps; without
+    convergence it stops at max_steps and reports converged=False."""
+    s = slit("textbook", 0.1667, 1e-5, "f32")
+    tol, interval = 1e-6, 300
+    steps, ok, hist = s.run_to_steady(tol=tol, interval=interval, max_steps=30000)
+    assert ok and steps == interval * len(hist)
+    rel = [abs(b - a) / abs(b) for a, b in zip(hist[:-1], hist[1:])]
+    assert rel[-1] < tol and all(r >= tol for r in rel[:-1])
+    # <vx> is the mean over the fluid list of the reported v (SP:385-388)
+    assert abs(hist[-1] - s.v.to_numpy()[:, :, 1:-1, 0].mean()) < 1e-6 * abs(hist[-1])
+    s2 = slit("textbook", 0.1667, 1e-5, "f32")
+    steps2, ok2, hist2 = s2.run_to_steady(tol=1e-12, interval=interval, max_steps=1000)
+    assert not ok2 and steps2 == 1000 and len(hist2) == 4
+
+
+def test_pack256_pressure_faces_and_slab_flux(gpu):
+    """The benchmark workload at full size (256^3 pack, seed 0): after the run
+    to steady state every fluid node of the two pressure faces carries
+    feq(rho_bc, 0), so rho = rho_bc and v = 0 (SP:274-281, :290-297: v_prev at
+    a face node starts at 0 and the feq populations cancel in pairs); and the
+    x mass flux per slab, sum rho v_x over 1 <= i <= nx-2, is the same in every
+    slab up to a spread that shrinks as the run approaches steady state."""
+    import torch
+    from taichi_lbm3d_amd.geometry import sphere_pack
+    solid = sphere_pack((256,) * 3, porosity=0.35, rmin=6.0, rmax=14.0, seed=0, periodic=(False, True, True))
+    s = pressure_solver(solid, "f32", wave=16)
+    cells = s._dev["cells"][: s.n_fluid].long()
+    i = cells // (256 * 256)
+
+    def slab_flux():
+        rho, v = s.compact_macros()
+        n = s.n_fluid
+        j = rho[:n].double() * v[:n].double()
+        phi = torch.zeros(256, dtype=torch.float64, device=j.device).index_add_(0, i, j)
+        inner = phi[1:255]
+        return float((inner.max() - inner.min()) / inner.mean()), rho[:n], v[: 3 * n].view(3, n)
+
+    s.run(6000)
+    spread_a, _, _ = slab_flux()
+    steps, ok, _ = s.run_to_steady(tol=1e-6, interval=500, max_steps=30000)
+    spread_b, rho, v = slab_flux()
+    print(f"steady after {steps} steps ({ok}); slab flux spread {spread_a:.2e} at 6000, {spread_b:.2e} at {steps}")
+    assert ok
+    for face, r in ((0, 1.0005), (255, 1.0)):
+        m = i == face
+        assert int(m.sum()) > 1000
+        assert torch.all(v[:, m] == 0)
+        assert float((rho[m] - r).abs().max()) <= 2.0 ** -22
+    assert spread_b < spread_a and spread_b < 2e-3
+
+
+def test_freezing_follows_taichi(gpu):
+    """What the reference freezes and when (SP:118-149 at init_simulation; the
+    force, face modes and face densities when the kernels first compile, at
+    the first step): niu changed after init_simulation is ignored; fx set
+    after init_simulation with force_flag 0 only shifts the reported v
+    (SP:385-386); a face mode or density changed before the first step
+    counts, with the face velocity frozen at init_simulation; anything changed
+    after the first step is ignored.  The same sequence of attribute changes
+    on the GPU solver and the oracle gives the same fields."""
+    from taichi_lbm3d_amd import LB3D_Solver_Single_Phase
+    rng = np.random.default_rng(11)
+    shape = (10, 9, 8)
+    solid = (rng.random(shape) < 0.2).astype(np.int8)
+
+    def script(obj, is_oracle):
+        def rho_face(face, val):
+            if is_oracle:
+                obj.set_bc_rho(face, val)
+            else:
+                getattr(obj, "set_bc_rho_" + {"x_left": "x0", "x_right": "x1", "y_left": "y0"}[face])(val)
+
+        def vel_face(face, val):
+            if is_oracle:
+                obj.set_bc_vel(face, val)
+            else:
+                getattr(obj, "set_bc_vel_" + {"z_left": "z0", "z_right": "z1"}[face])(val)
+
+        obj.niu = 0.1
+        rho_face("x_left", 1.004)
+        vel_face("z_right", [0.02, 0.0, 0.0])
+        yield
+        obj.niu = 0.4                        # after init_simulation: ignored
+        obj.fx = 2e-4                        # force_flag stays 0: only v's F/2 shift
+        rho_face("x_right", 0.997)           # new pressure face before the first step: counts
+        rho_face("x_left", 1.002)            # new density before the first step: counts
+        if is_oracle:                        # velocity changed after init_simulation: ignored
+            obj.u_bc[FACES.index("z_right")] = [0.09, 0.0, 0.0]
+        else:
+            obj.vx_bczr = 0.09
+        yield
+        obj.niu = 0.3                        # after the first step: all ignored
+        obj.fx = 0.0
+        rho_face("y_left", 1.01)
+        yield
+
+    s = LB3D_Solver_Single_Phase(*shape, sparse_storage=False)
+    s.solid.from_numpy(solid)
+    o = COracle(*shape, real="f64", threads=THREADS)
+    o.set_solid(solid)
+    ss, os_ = script(s, False), script(o, True)
+    next(ss), next(os_)
+    s.init_simulation()
+    o.init_simulation()
+    next(ss), next(os_)
+    s.run(5)
+    o.step(5)
+    next(ss), next(os_)
+    s.run(20)
+    o.step(20)
+    fl = solid == 0
+    rho, v = s.rho.to_numpy(), s.v.to_numpy()
+    assert rel_l2(rho[fl], o.rho()[fl]) < 1e-6 and rel_l2(v[fl], o.v()[fl]) < 1e-5
+    # the changes did take effect: the right x face is a pressure face at 0.997
+    # (away from its edge with the z-right velocity face, which owns that edge)
+    face = fl[-1][:, :-1]
+    assert np.abs(rho[-1][:, :-1][face] - 0.997).max() < 1e-6
+    assert s.force_flag == 0 and abs(s.s_v - 1.0 / (0.1 / 3 + 0.5)) < 1e-15
+
+
+@pytest.mark.parametrize("dtype", ["f32", "f16sr"])
+def test_wave_order_is_bit_exact(gpu, dtype):
+    """lbm3d_run_wave (chunks of planes, each odd update right after the even
+    update of the chunk beyond it) leaves exactly the populations and face
+    state of one launch per step, from even and odd starting steps."""
+    from taichi_lbm3d_amd.geometry import sphere_pack
+    solid = sphere_pack((96, 40, 48), porosity=0.4, rmin=3.0, rmax=7.0, seed=5, periodic=(False, True, True))
+    runs = {}
+    for wave in (0, 3, 7, 24):
+        s = pressure_solver(solid, dtype, wave=wave)
+        s.run(7)
+        s.run(9)
+        s.run(4)
+        runs[wave] = s
+        if wave:
+            assert 3 <= s._wave_plan.n_chunks <= wave
+    a = runs[0]
+    for wave in (3, 7, 24):
+        b = runs[wave]
+        assert b.steps_done == a.steps_done == 20
+        assert np.array_equal(a._dev["g"].cpu().numpy(), b._dev["g"].cpu().numpy()), wave
+        assert np.array_equal(a._dev["face_u"].cpu().numpy(), b._dev["face_u"].cpu().numpy()), wave
