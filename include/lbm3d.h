@@ -258,11 +258,13 @@ int lbm3d_scatter_dense(void *stream, const lbm3d_params *p, const int32_t *cell
                         int32_t oz, float *rho_dense, float *v_dense);
 
 /* FP64 reductions over the compact fields (cal_max_v, SP:399-402, plus the
- * flux sums used for permeability and convergence):
+ * flux and density sums used for permeability and convergence):
  *   out[0] sum rho        out[1..3] sum v       out[4] max |v|
  *   out[5] count of non-finite rho or v         out[6] sum vx over i_lo <= i <= i_hi
- *   out[7] number of cells in that slab range
- * `out` is device double[8]; workspace of lbm3d_reduce_workspace_bytes(). */
+ *   out[7] number of cells in that slab range   out[8] sum rho over that slab range
+ *   out[9] 0
+ * Cells with a non-finite value count in out[5] only.  `out` is device
+ * double[10]; workspace of lbm3d_reduce_workspace_bytes(). */
 size_t lbm3d_reduce_workspace_bytes(int32_t n_cells);
 int lbm3d_reduce(void *stream, const lbm3d_params *p, const int32_t *cells, const float *rho,
                  const float *v, int32_t i_lo, int32_t i_hi, double *out, void *workspace,

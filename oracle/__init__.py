@@ -37,20 +37,36 @@ def build(force: bool = False) -> None:
         subprocess.run(["make", "-s", "-B" if force else "all"], cwd=_HERE, check=True)
 
 
+def build_native() -> str:
+    """Compile the CPU baseline's build of the FP32 restatement (-O3
+    -march=native) for the host this runs on; returns the compiler flags."""
+    subprocess.run(["make", "-s", "-B", "native"], cwd=_HERE, check=True)
+    out = subprocess.run(["make", "-s", "-p", "-n", "native"], cwd=_HERE, capture_output=True, text=True).stdout
+    for line in out.splitlines():
+        if line.startswith("CFLAGS_NATIVE"):
+            return line.split("=", 1)[1].strip()
+    return ""
+
+
 _LIBS: dict = {}
+# real -> (library file, symbol prefix); "f32native" is the CPU baseline's
+# host-tuned FP32 build (build_native), never the checker
+_FILES = {"f32": ("liblbm_oracle32.so", "orc32_"), "f64": ("liblbm_oracle64.so", "orc64_"),
+          "f32native": ("liblbm_oracle32n.so", "orc32_")}
 
 
 def _lib(real: str):
-    if real not in ("f32", "f64"):
-        raise ValueError("real must be 'f32' or 'f64'")
+    if real not in _FILES:
+        raise ValueError(f"real must be one of {sorted(_FILES)}")
     if real in _LIBS:
         return _LIBS[real]
-    name = "liblbm_oracle32.so" if real == "f32" else "liblbm_oracle64.so"
+    name, p = _FILES[real]
     path = os.path.join(_HERE, name)
     if not os.path.exists(path):
+        if real == "f32native":
+            raise FileNotFoundError(f"{path}: call oracle.build_native() on this host first")
         build()
     lib = ctypes.CDLL(path)
-    p = "orc32_" if real == "f32" else "orc64_"
     vp, ip, dp = ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(ctypes.c_double)
 
     def fn(n, res, args):

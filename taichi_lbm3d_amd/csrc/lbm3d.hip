@@ -1420,24 +1420,26 @@ __global__ void __launch_bounds__(kBlock) k_scatter(const KP kp, const int32_t *
 
 // FP64 reductions: per-block partials, then one wave per quantity folds them
 // in a fixed order (the result does not depend on scheduling).
-constexpr int kRed = 8;
+constexpr int kRed = 8;       // quantities per batch sample
+constexpr int kRedGrid = 10;  // quantities of a single-grid reduction
 
 __device__ __forceinline__ double fold(int r, double a, double b) {
   return r == 4 ? (b > a ? b : a) : a + b;  // quantity 4 is a maximum, the others sums
 }
 
-// sums over a block: acc[r] of every thread -> out[r] (valid in thread 0..kRed-1)
-__device__ __forceinline__ void block_fold(double acc[kRed], double *__restrict__ out) {
-  __shared__ double s[kRed][kBlock / 64];
+// sums over a block: acc[r] of every thread -> out[r] (valid in thread 0..N-1)
+template <int N>
+__device__ __forceinline__ void block_fold(double acc[N], double *__restrict__ out) {
+  __shared__ double s[N][kBlock / 64];
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
 #pragma unroll
-  for (int r = 0; r < kRed; ++r) {
+  for (int r = 0; r < N; ++r) {
     double val = acc[r];
     for (int d = 32; d > 0; d >>= 1) val = fold(r, val, __shfl_down(val, d, 64));
     if (lane == 0) s[r][wid] = val;
   }
   __syncthreads();
-  if (threadIdx.x < kRed) {
+  if (threadIdx.x < N) {
     const int r = threadIdx.x;
     double val = s[r][0];
     for (int w = 1; w < kBlock / 64; ++w) val = fold(r, val, s[r][w]);
@@ -1445,7 +1447,8 @@ __device__ __forceinline__ void block_fold(double acc[kRed], double *__restrict_
   }
 }
 
-__device__ __forceinline__ bool accumulate(double acc[kRed], float r, float a, float b, float c) {
+// quantities 0-5: sum rho, sum v (3), max |v|, non-finite cells
+__device__ __forceinline__ bool accumulate(double *acc, float r, float a, float b, float c) {
   const bool fin = isfinite(r) && isfinite(a) && isfinite(b) && isfinite(c);
   if (!fin) {
     acc[5] += 1.0;
@@ -1460,32 +1463,34 @@ __device__ __forceinline__ bool accumulate(double acc[kRed], float r, float a, f
   return true;
 }
 
+// 6-8: sum vx, cells and sum rho over the slabs i_lo <= i <= i_hi; 9: 0
 __global__ void __launch_bounds__(kBlock) k_reduce_partial(const KP kp, const int32_t *__restrict__ cells,
                                                            const float *__restrict__ rho,
                                                            const float *__restrict__ v, int32_t i_lo,
                                                            int32_t i_hi, double *__restrict__ part) {
   const uint32_t N = (uint32_t)kp.p.n_cells;
-  double acc[kRed] = {0, 0, 0, 0, 0, 0, 0, 0};
+  double acc[kRedGrid] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
   for (uint32_t x = blockIdx.x * kBlock + threadIdx.x; x < N; x += gridDim.x * kBlock) {
-    const float a = v[x];
-    if (!accumulate(acc, rho[x], a, v[(size_t)N + x], v[2 * (size_t)N + x])) continue;
+    const float a = v[x], r = rho[x];
+    if (!accumulate(acc, r, a, v[(size_t)N + x], v[2 * (size_t)N + x])) continue;
     const int i = (int)fdiv((uint32_t)cells[x], kp.geo.m_nyz, kp.geo.s_nyz);
     if (i >= i_lo && i <= i_hi) {
       acc[6] += a;
       acc[7] += 1.0;
+      acc[8] += r;
     }
   }
-  double out[kRed];
-  block_fold(acc, out);
-  if (threadIdx.x < kRed) part[(size_t)blockIdx.x * kRed + threadIdx.x] = out[threadIdx.x];
+  double out[kRedGrid];
+  block_fold<kRedGrid>(acc, out);
+  if (threadIdx.x < kRedGrid) part[(size_t)blockIdx.x * kRedGrid + threadIdx.x] = out[threadIdx.x];
 }
 
 // one wave per quantity r: lanes stride over the partials, then a shuffle tree
-__global__ void __launch_bounds__(64 * kRed) k_reduce_final(const double *__restrict__ part, int nparts,
-                                                           double *__restrict__ out) {
+__global__ void __launch_bounds__(64 * kRedGrid) k_reduce_final(const double *__restrict__ part, int nparts,
+                                                               double *__restrict__ out) {
   const int r = threadIdx.x >> 6, lane = threadIdx.x & 63;
   double val = 0.0;
-  for (int b = lane; b < nparts; b += 64) val = fold(r, val, part[(size_t)b * kRed + r]);
+  for (int b = lane; b < nparts; b += 64) val = fold(r, val, part[(size_t)b * kRedGrid + r]);
   for (int d = 32; d > 0; d >>= 1) val = fold(r, val, __shfl_down(val, d, 64));
   if (lane == 0) out[r] = val;
 }
@@ -1504,7 +1509,7 @@ __global__ void __launch_bounds__(kBlock) k_reduce_batch_partial(const Batch bt,
     acc[6] += 1.0;
   }
   double out[kRed];
-  block_fold(acc, out);
+  block_fold<kRed>(acc, out);
   if (threadIdx.x < kRed) part[(size_t)c * kRed + threadIdx.x] = out[threadIdx.x];
 }
 
@@ -2126,7 +2131,7 @@ int lbm3d_scatter_dense(void *stream, const lbm3d_params *p, const int32_t *cell
 
 size_t lbm3d_reduce_workspace_bytes(int32_t n_cells) {
   (void)n_cells;
-  return (size_t)kRedBlocks * kRed * sizeof(double);
+  return (size_t)kRedBlocks * kRedGrid * sizeof(double);
 }
 
 int lbm3d_reduce(void *stream, const lbm3d_params *p, const int32_t *cells, const float *rho, const float *v,
@@ -2144,7 +2149,7 @@ int lbm3d_reduce(void *stream, const lbm3d_params *p, const int32_t *cells, cons
   hipStream_t s = (hipStream_t)stream;
   k_reduce_partial<<<blocks, kBlock, 0, s>>>(kp, cells, rho, v, i_lo, i_hi, (double *)workspace);
   if ((rc = check_launch("lbm3d_reduce"))) return rc;
-  k_reduce_final<<<1, 64 * kRed, 0, s>>>((const double *)workspace, blocks, out);
+  k_reduce_final<<<1, 64 * kRedGrid, 0, s>>>((const double *)workspace, blocks, out);
   return check_launch("lbm3d_reduce/final");
 }
 

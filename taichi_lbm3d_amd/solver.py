@@ -532,7 +532,8 @@ class LB3D_Solver_Single_Phase:
 
     def reduce(self, i_lo: int = 0, i_hi: Optional[int] = None):
         """FP64 sums over the fluid list: dict of sum_rho, sum_v (3), max_v,
-        nonfinite, slab_sum_vx (i_lo <= i <= i_hi) and slab_count."""
+        nonfinite, and over the slabs i_lo <= i <= i_hi slab_sum_vx,
+        slab_count and slab_sum_rho."""
         torch = _torch()
         p = self._params
         dev = self._torch_device()
@@ -540,13 +541,13 @@ class LB3D_Solver_Single_Phase:
         lib = _lib.lib()
         ws_bytes = lib.size("lbm3d_reduce_workspace_bytes", p.n_cells)
         ws = torch.empty(ws_bytes, dtype=torch.uint8, device=dev)
-        out = torch.zeros(8, dtype=torch.float64, device=dev)
+        out = torch.zeros(10, dtype=torch.float64, device=dev)
         with torch.cuda.device(dev):
             lib.call("lbm3d_reduce", self._stream(), _ref(p), _ptr(self._dev["cells"]), _ptr(rho_c), _ptr(v_c),
                      int(i_lo), int(self.nx - 1 if i_hi is None else i_hi), _ptr(out), _ptr(ws), ws_bytes)
         o = out.cpu().numpy()
         return {"sum_rho": o[0], "sum_v": o[1:4].copy(), "max_v": o[4], "nonfinite": int(o[5]),
-                "slab_sum_vx": o[6], "slab_count": int(o[7])}
+                "slab_sum_vx": o[6], "slab_count": int(o[7]), "slab_sum_rho": o[8]}
 
     def get_max_v(self):
         """max |v| over the domain (cal_max_v, SP:394-402)."""
@@ -561,12 +562,22 @@ class LB3D_Solver_Single_Phase:
         tau, _, _ = relaxation_rates(float(self.niu), self.convention)
         return (tau - 0.5) / 3.0
 
-    def permeability(self) -> float:
+    def permeability(self, method: str = "faces") -> float:
         """Darcy permeability along x in lattice units^2.
 
-        Pressure drive (x faces in mode 1):
+        Pressure drive (x faces in mode 1), method "faces":
             k = nu_eff * q * (nx - 1) / ((rho_left - rho_right) / 3),
-        q the mean superficial x velocity over the slabs 1 <= i <= nx-2.
+        q the mean superficial x velocity over the slabs 1 <= i <= nx-2: the
+        apparent permeability of the whole sample between the two face
+        densities.  The reference's pressure faces hold their nodes at
+        feq(rho_bc, 0) (SP:274-297), so part of the density drop sits at the
+        faces themselves and this k includes their resistance.
+        Method "gradient" takes the drop inside the sample instead:
+            k = rho_m * nu_eff * q / G,  G = (rho_a - rho_b) / (3 (b - a)),
+        rho_a, rho_b the mean density of the fluid cells of the slabs
+        a = nx // 4 and b = nx - 1 - nx // 4, q the superficial mean x velocity
+        and rho_m the mean density over the slabs a .. b (dynamic viscosity
+        rho nu).
         Body force (fx != 0):
             k = nu_eff * q / F_eff,  F_eff = a * fx / 3 the momentum the
         force term injects per step (a = 1/3 for the reference convention),
@@ -575,6 +586,22 @@ class LB3D_Solver_Single_Phase:
         """
         p = self._params
         nu = self.effective_viscosity()
+        if method not in ("faces", "gradient"):
+            raise ValueError("method must be 'faces' or 'gradient'")
+        if p.bc_mode[0] == 1 and p.bc_mode[1] == 1 and method == "gradient":
+            a, b = self.nx // 4, self.nx - 1 - self.nx // 4
+            if b - a < 1:
+                raise ValueError("the gradient method needs nx >= 4")
+            ra, rb = self.reduce(a, a), self.reduce(b, b)
+            if ra["slab_count"] == 0 or rb["slab_count"] == 0:
+                raise ValueError("a slab of the gradient method holds no fluid cell")
+            r = self.reduce(a, b)
+            q = r["slab_sum_vx"] / (self.ny * self.nz * (b - a + 1))
+            rho_m = r["slab_sum_rho"] / r["slab_count"]
+            g = (ra["slab_sum_rho"] / ra["slab_count"] - rb["slab_sum_rho"] / rb["slab_count"]) / (3.0 * (b - a))
+            return rho_m * nu * q / g
+        if method == "gradient":
+            raise ValueError("the gradient method needs pressure faces on both x faces")
         if p.bc_mode[0] == 1 and p.bc_mode[1] == 1:
             r = self.reduce(1, self.nx - 2)
             q = r["slab_sum_vx"] / (self.ny * self.nz * max(self.nx - 2, 1))

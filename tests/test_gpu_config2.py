@@ -122,6 +122,32 @@ def test_slit_permeability_reference_scaling(gpu):
     assert abs(k / kc - 1) < 1.5 * drift + 1e-9
 
 
+@pytest.mark.parametrize("conv,niu", [("textbook", 0.1), ("reference", 0.5)])
+def test_pressure_slit_gradient_permeability(gpu, conv, niu):
+    """Pressure-driven slit (64 x 4 x 16, solid planes z = 0 and NZ-1,
+    pressure x faces 1.0005 / 1.0): the flow is the same lattice-exact parabola
+    driven by the interior pressure gradient, so permeability("gradient")
+    equals the closed form (1e-4 covers the remaining compressibility and the
+    face disturbance at slabs nx/4 and 3nx/4, 6e-5 in the FP64 oracle).  The
+    faces' own resistance makes the apparent permeability("faces") several
+    times smaller here."""
+    from taichi_lbm3d_amd import LB3D_Solver_Single_Phase
+    solid = np.zeros((64, 4, 16), np.int8)
+    solid[:, :, 0] = solid[:, :, -1] = 1
+    s = LB3D_Solver_Single_Phase(64, 4, 16, convention=conv, dtype="f64")
+    s.solid.from_numpy(solid)
+    s.niu = niu
+    s.set_bc_rho_x0(1.0005)
+    s.set_bc_rho_x1(1.0)
+    s.init_simulation()
+    steps, ok, _ = s.run_to_steady(tol=1e-11, interval=500, max_steps=60000)
+    kg, kf, kc = s.permeability("gradient"), s.permeability(), slit_k_closed()
+    print(conv, steps, ok, "gradient", kg / kc - 1, "faces", kf / kc)
+    assert ok
+    assert abs(kg / kc - 1) < 1e-4
+    assert kf < 0.5 * kg
+
+
 def test_run_to_steady_stops_by_its_criterion(gpu):
     """run_to_steady returns at the first check whose relative change of <vx>
     against the previous check is below tol, every `interval` steps; without
