@@ -2,7 +2,8 @@
 // declared in include/lbm3d_io.h.
 //
 // lbm3d_read_geo_ascii parses the reference's voxel file (one number per line,
-// x fastest, SP:173-177) on the host with a hand-written number scanner.
+// x fastest, SP:173-177) on the host with a hand-written number scanner
+// (geo_ascii.h, plain C++ shared with the sanitizer build).
 //
 // lbm3d_voxelize flags grid nodes inside a closed triangle mesh, the job of the
 // reference's Convert_stl_to_binary.cpp:29-157 (libigl's signed distance with
@@ -21,6 +22,7 @@
 #include <cstdlib>
 #include <cstring>
 
+#include "geo_ascii.h"
 #include "lbm3d.h"
 #include "lbm3d_io.h"
 
@@ -77,40 +79,6 @@ __global__ void __launch_bounds__(256)
   }
 }
 
-// next number of a whitespace-separated ASCII stream; false at end of input
-bool next_number(const char *&p, const char *end, double &val) {
-  while (p < end && (*p == ' ' || *p == '\n' || *p == '\r' || *p == '\t' || *p == ',')) ++p;
-  if (p >= end) return false;
-  // fast path: an optionally signed integer followed by a separator
-  const char *q = p;
-  bool neg = false;
-  if (*q == '-' || *q == '+') neg = *q++ == '-';
-  int64_t acc = 0;
-  const char *digits = q;
-  while (q < end && *q >= '0' && *q <= '9' && q - digits < 18) acc = acc * 10 + (*q++ - '0');
-  if (q > digits && (q == end || *q == ' ' || *q == '\n' || *q == '\r' || *q == '\t' || *q == ',')) {
-    val = neg ? -(double)acc : (double)acc;
-    p = q;
-    return true;
-  }
-  // general number (fractions, exponents): strtod on a bounded copy
-  char buf[64];
-  size_t len = 0;
-  while (p + len < end && len < sizeof(buf) - 1 && p[len] != ' ' && p[len] != '\n' && p[len] != '\r' &&
-         p[len] != '\t' && p[len] != ',')
-    ++len;
-  memcpy(buf, p, len);
-  buf[len] = 0;
-  char *stop = nullptr;
-  errno = 0;
-  val = strtod(buf, &stop);
-  if (stop == buf) {
-    val = NAN;  // not a number: reported by the caller
-  }
-  p += len;
-  return true;
-}
-
 }  // namespace
 
 extern "C" {
@@ -118,52 +86,7 @@ extern "C" {
 const char *lbm3d_io_last_error(void) { return io_err; }
 
 int64_t lbm3d_read_geo_ascii(const char *path, int32_t nx, int32_t ny, int32_t nz, int8_t *solid) {
-  if (!path || !solid || nx < 1 || ny < 1 || nz < 1) return iofail(LBM3D_EINVAL, "bad argument to lbm3d_read_geo_ascii");
-  FILE *f = fopen(path, "rb");
-  if (!f) return iofail(LBM3D_EINVAL, "cannot open ", path);
-  if (fseek(f, 0, SEEK_END) != 0) {
-    fclose(f);
-    return iofail(LBM3D_EINVAL, "cannot seek ", path);
-  }
-  const long size = ftell(f);
-  if (size < 0) {  // not a seekable regular file
-    fclose(f);
-    return iofail(LBM3D_EINVAL, "cannot size ", path);
-  }
-  rewind(f);
-  const size_t cap = size > 0 ? (size_t)size : 1;
-  char *buf = (char *)malloc(cap);
-  if (!buf) {
-    fclose(f);
-    return iofail(LBM3D_ESIZE, "out of host memory reading ", path);
-  }
-  const size_t got = fread(buf, 1, (size_t)size, f);  // size <= cap
-  fclose(f);
-  const int64_t total = (int64_t)nx * ny * nz;
-  const char *p = buf, *end = buf + got;
-  int64_t count = 0;
-  double v;
-  while (next_number(p, end, v)) {
-    if (std::isnan(v)) {
-      free(buf);
-      return iofail(LBM3D_EINVAL, "not a number in ", path);
-    }
-    if (count < total) {
-      // value n of the file is cell (i, j, k) with n = i + nx*(j + ny*k) (x fastest);
-      // stored in C order; values > 0 become 1, others truncate to int8
-      const int64_t i = count % nx, t = count / nx;
-      const int64_t j = t % ny, k = t / ny;
-      const double c = v > 0 ? 1.0 : std::trunc(v);
-      solid[(i * ny + j) * nz + k] = (int8_t)(int)c;
-    }
-    ++count;
-  }
-  free(buf);
-  if (count != total) {
-    snprintf(io_err, sizeof(io_err), "%s: %lld values, expected %lld", path, (long long)count, (long long)total);
-    return LBM3D_EINVAL;
-  }
-  return count;
+  return lbm3d_ascii::read_geo(path, nx, ny, nz, solid, io_err, sizeof(io_err));
 }
 
 int lbm3d_voxelize(void *stream, const float *tris, int32_t n_tris, double x0, double y0, double z0, double dx,
