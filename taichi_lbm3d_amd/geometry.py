@@ -34,12 +34,6 @@ def write_geo(path: str, solid: np.ndarray) -> None:
     np.savetxt(path, flat.astype(np.int64), fmt="%d")
 
 
-def _ball_offsets(r: float):
-    ri = int(np.ceil(r))
-    ax = np.arange(-ri, ri + 1)
-    return ri, ax
-
-
 def sphere_pack(shape=(256, 256, 256), porosity: float = 0.35, rmin: float = 6.0, rmax: float = 14.0,
                 seed: int = 0, periodic=(False, True, True), max_spheres: int = 10_000_000) -> np.ndarray:
     """Seeded random overlapping-sphere pack.
@@ -49,47 +43,64 @@ def sphere_pack(shape=(256, 256, 256), porosity: float = 0.35, rmin: float = 6.0
     minimum-image wrap along the axes marked periodic and clipping along the
     others.  Spheres are added one at a time until the fluid fraction is at or
     below ``porosity``.
+
+    Sphere s draws four doubles from ``default_rng(seed)``: the centre
+    (u0, u1, u2) * shape and the radius rmin + (rmax - rmin) u3.  Cell (i,j,k)
+    lies in the sphere when ((0 + dx^2) + dy^2) + dz^2 <= r*r, d the offset of
+    the cell (before wrapping) from the centre.  The spheres are evaluated in
+    chunks: within a chunk, each newly covered cell is credited to the first
+    sphere covering it, and the pack keeps the spheres up to the first one at
+    which the solid count reaches the target -- the same cells as adding them
+    one at a time.
     """
-    nx, ny, nz = shape
-    n = np.array(shape)
+    nx, ny, nz = (int(a) for a in shape)
+    n = np.array([nx, ny, nz], np.float64)
     rng = np.random.default_rng(seed)
-    solid = np.zeros(shape, np.uint8)
-    total = nx * ny * nz
+    flat = np.zeros(nx * ny * nz, np.uint8)
+    target = (1.0 - porosity) * flat.size
     n_solid = 0
-    target = (1.0 - porosity) * total
-    for _ in range(max_spheres):
-        if n_solid >= target:
-            break
-        c = rng.uniform(0.0, 1.0, 3) * n
-        r = rng.uniform(rmin, rmax)
-        ri = int(np.ceil(r))
-        idx = []
-        dist2 = 0.0
-        for a in range(3):
-            lo = int(np.floor(c[a] - r))
-            hi = int(np.ceil(c[a] + r))
-            cells = np.arange(lo, hi + 1)
-            d = cells - c[a]
+    box = 2 * int(np.ceil(rmax)) + 3  # cells per axis of a sphere's bounding range
+    # spheres per chunk: about a quarter of the count a pack without overlap would
+    # need, at most ~4M candidate cells per chunk
+    ball = 4.0 / 3.0 * np.pi * (0.5 * (rmin + rmax)) ** 3
+    chunk = int(max(8, min(target / ball / 4 + 1, 4_000_000 // box ** 3)))
+    off = np.arange(box, dtype=np.int64)
+    used = 0
+    while n_solid < target and used < max_spheres:
+        k = min(chunk, max_spheres - used)
+        u = rng.random((k, 4))
+        cen = u[:, :3] * n
+        r = rmin + (rmax - rmin) * u[:, 3]
+        axes = []
+        for a, ext in enumerate((nx, ny, nz)):
+            lo = np.floor(cen[:, a] - r).astype(np.int64)
+            hi = np.ceil(cen[:, a] + r).astype(np.int64)
+            cells = lo[:, None] + off[None, :]
+            d = cells - cen[:, a:a + 1]
+            ok = cells <= hi[:, None]
             if periodic[a]:
-                cells = np.mod(cells, n[a])
+                cells = np.mod(cells, ext)
             else:
-                keep = (cells >= 0) & (cells < n[a])
-                cells, d = cells[keep], d[keep]
-            shp = [1, 1, 1]
-            shp[a] = -1
-            idx.append(cells.reshape(shp))
-            dist2 = dist2 + (d * d).reshape(shp)
-        inside = dist2 <= r * r
-        del ri
-        sub = solid[idx[0], idx[1], idx[2]]
-        # cells may repeat along a periodic axis only when 2r+1 > n; count once
-        newly = inside & (sub == 0)
-        if np.any(newly):
-            ii, jj, kk = np.broadcast_arrays(idx[0], idx[1], idx[2])
-            sel = np.unique(np.stack([ii[newly], jj[newly], kk[newly]], 0), axis=1)
-            solid[sel[0], sel[1], sel[2]] = 1
-            n_solid += sel.shape[1]
-    return solid
+                ok &= (cells >= 0) & (cells < ext)
+            axes.append((cells, d * d, ok))
+        (cx, dx2, okx), (cy, dy2, oky), (cz, dz2, okz) = axes
+        dist2 = ((0.0 + dx2)[:, :, None, None] + dy2[:, None, :, None]) + dz2[:, None, None, :]
+        inside = (dist2 <= (r * r)[:, None, None, None]) & okx[:, :, None, None] & oky[:, None, :, None] \
+            & okz[:, None, None, :]
+        sid, a_, b_, c_ = np.nonzero(inside)
+        idx = (cx[sid, a_] * ny + cy[sid, b_]) * nz + cz[sid, c_]
+        fresh = flat[idx] == 0
+        sid, idx = sid[fresh], idx[fresh]
+        # first sphere of the chunk covering each cell (nonzero order is by sphere)
+        uidx, first = np.unique(idx, return_index=True)
+        owner = sid[first]
+        gained = np.cumsum(np.bincount(owner, minlength=k))
+        reach = np.nonzero(n_solid + gained >= target)[0]
+        last = int(reach[0]) if reach.size else k - 1
+        flat[uidx[owner <= last]] = 1
+        n_solid += int(gained[last])
+        used += last + 1
+    return flat.reshape(nx, ny, nz)
 
 
 def cube_channel(shape=(512, 512, 512), fraction: float = 0.05, emin: int = 4, emax: int = 16,

@@ -166,6 +166,120 @@ def train(net, sampler, iters, batch, crop, lr=1e-3, log_every=50, border=sg.BOR
     return [(i, l) for i, l in enumerate(losses) if l is not None], time.perf_counter() - t0
 
 
+def training_data(solids, max_steps=50_000):
+    """The batched solver's converged fields of `solids` as network inputs and
+    targets; returns (inputs, targets, k_train, nu, info)."""
+    import torch
+    dev = torch.device("cuda", torch.cuda.current_device())
+    b, steps, conv, wall, _ = converge(solids, max_steps=max_steps)
+    log(f"training data: {int(conv.sum())}/{len(conv)} converged, steps {steps.min()}-{steps.max()}, {wall:.1f}s")
+    nu = b.effective_viscosity()
+    k_train = b.permeability()
+    fields = [b.sample_fields(s) for s in range(b.n_samples)]
+    del b
+    torch.cuda.empty_cache()
+    inputs = [sg.make_inputs(s, device=dev) for s in solids]
+    targets = [sg.make_targets(r, v, s, FX, nu) for (r, v), s in zip(fields, solids)]
+    info = {"samples": len(solids), "converged": int(conv.sum()), "steps_max": int(steps.max()),
+            "solver_wall_s": round(wall, 2)}
+    return inputs, targets, k_train, nu, info
+
+
+def velocity_scale(net, solids, k_train, nu):
+    """Median ratio of the training samples' converged permeability to the
+    permeability of the network's own warm start at t = 0."""
+    k_tr_pred = initial_permeability(net, solids, nu)
+    scale = float(np.median(k_train / k_tr_pred))
+    log(f"calibration: velocity scale {scale:.4f} (training k ratio {np.min(k_train / k_tr_pred):.3f}"
+        f"-{np.max(k_train / k_tr_pred):.3f})")
+    return scale
+
+
+def evaluate(net, ev, nu, scale=1.0, density=False, max_steps=50_000):
+    """Cold and warm starts of the packs `ev`, each batch run to the 1e-6
+    steady criterion; returns the report dict (steps, wall-clock with the warm
+    side including inference, permeability errors, steps and time to within
+    1 % of the cold permeability, fallbacks) with the per-sample arrays under
+    "per_sample"."""
+    import torch
+    dev = torch.device("cuda", torch.cuda.current_device())
+    bc, steps_c, conv_c, wall_c, hist_c = converge(ev, max_steps=max_steps)
+    k_cold = bc.permeability()
+    ref = [bc.sample_fields(s) for s in range(bc.n_samples)]
+    del bc
+    torch.cuda.empty_cache()
+    log(f"cold: {int(conv_c.sum())} converged, steps median {np.median(steps_c):.0f}, {wall_c:.1f}s")
+
+    # the geometry goes to the device once, as the solver's own copy does, before the clock
+    ev_dev = [torch.from_numpy(np.ascontiguousarray(s)).to(dev) for s in ev]
+    torch.cuda.synchronize()
+    t_inf = time.perf_counter()
+    preds = sg.predict(net, ev_dev, FX, nu, density=density)
+    if scale != 1.0:
+        preds = [(r, v * scale) for r, v in preds]
+    torch.cuda.synchronize()
+    inf_s = time.perf_counter() - t_inf
+    fallback = np.array([not (torch.isfinite(r).all() and torch.isfinite(v).all()) for r, v in preds])
+    preds = [(torch.ones_like(r), torch.zeros_like(v)) if fb else (r, v) for (r, v), fb in zip(preds, fallback)]
+    # the surrogate's error, judged by the converged cold-start fields
+    v_err, rho_err = [], []
+    for (r, v), (rr, vr), s in zip(preds, ref, ev):
+        fl = s == 0
+        vp = v.cpu().numpy()
+        v_err.append(float(np.linalg.norm(vp[fl] - vr[fl]) / np.linalg.norm(vr[fl])))
+        # density deviation from the sample's fluid mean, against the cold start's
+        dr, dp = rr[fl] - rr[fl].mean(), r.cpu().numpy()[fl] - 1.0
+        rho_err.append(float(np.linalg.norm(dp - dr) / np.linalg.norm(dr)))
+    try:
+        bw, steps_w, conv_w, wall_w, hist_w = converge(ev, max_steps=max_steps, init_fields=preds)
+        k_warm = bw.permeability()
+        del bw
+    except FloatingPointError as e:  # the warm run diverged: every sample falls back to the cold start
+        log(f"warm start diverged ({e}); falling back to cold start")
+        fallback[:] = True
+        steps_w, conv_w, wall_w, hist_w, k_warm = steps_c, conv_c, wall_c, hist_c, k_cold
+    fallback |= ~conv_w
+    k_warm = np.where(fallback, k_cold, k_warm)
+    k_pred = hist_w[0][2]
+    signed = (k_pred - k_cold) / np.abs(k_cold)
+    s1_c, w1_c = within(hist_c, k_cold)
+    s1_w, w1_w = within(hist_w, k_cold)
+    err = np.abs(k_warm - k_cold) / np.abs(k_cold)
+    perr = np.abs(k_pred - k_cold) / np.abs(k_cold)
+    return {
+        "samples": len(ev), "size": int(ev[0].shape[0]), "fallbacks": int(fallback.sum()),
+        "surrogate": {"v_rel_l2_median": float(np.median(v_err)), "v_rel_l2_max": float(np.max(v_err)),
+                      "rho_dev_rel_l2_median": float(np.median(rho_err)),
+                      "k_rel_err_median": float(np.median(perr)), "k_rel_err_max": float(np.max(perr)),
+                      "k_signed_err_median": float(np.median(signed)), "velocity_scale": scale,
+                      "inference_s": round(inf_s, 3)},
+        "cold": {"steps_median": float(np.median(steps_c)), "steps_max": int(steps_c.max()),
+                 "wall_s": round(wall_c, 2), "converged": int(conv_c.sum()),
+                 "steps_to_1pct_median": float(np.median(s1_c)), "steps_to_1pct_max": int(s1_c.max()),
+                 "wall_to_1pct_s": round(float(np.nanmax(w1_c)), 3)},
+        "warm": {"steps_median": float(np.median(steps_w)), "steps_max": int(steps_w.max()),
+                 "wall_s": round(wall_w + inf_s, 2), "converged": int(conv_w.sum()),
+                 "steps_to_1pct_median": float(np.median(s1_w)), "steps_to_1pct_max": int(s1_w.max()),
+                 "wall_to_1pct_s": round(float(np.nanmax(w1_w)) + inf_s, 3)},
+        "permeability_rel_err_warm_vs_cold": {"median": float(np.median(err)), "max": float(err.max())},
+        "speedup_to_steady": round(wall_c / (wall_w + inf_s), 3),
+        "speedup_to_1pct": round(float(np.nanmax(w1_c)) / (float(np.nanmax(w1_w)) + inf_s), 3),
+        # the same for the median sample: the batch's wall-clock when that
+        # sample came within 1 % (cold), against inference plus the same (warm)
+        "speedup_to_1pct_median_sample": round(float(np.nanmedian(w1_c)) / (float(np.nanmedian(w1_w)) + inf_s), 3),
+        "per_sample": {"k_cold": k_cold.tolist(), "k_warm": k_warm.tolist(), "steps_cold": steps_c.tolist(),
+                       "steps_warm": steps_w.tolist(), "steps_to_1pct_cold": s1_c.tolist(),
+                       "steps_to_1pct_warm": s1_w.tolist(), "fallback": fallback.tolist()},
+    }
+
+
+def eval_packs(n, size, seed=5000):
+    """The held-out packs: periodic, porosity 0.3, radii 4-12, seeds seed .. seed+n-1."""
+    from taichi_lbm3d_amd.geometry import sphere_pack
+    return [sphere_pack((size,) * 3, porosity=0.3, rmin=4.0, rmax=12.0, seed=seed + i, periodic=(True, True, True))
+            for i in range(n)]
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--train-samples", type=int, default=64)
@@ -194,23 +308,13 @@ def main(argv=None):
     log(f"{len(solids)} training geometries ({time.time() - t0:.1f}s)")
 
     import torch
-    from taichi_lbm3d_amd.geometry import sphere_pack
     assert torch.cuda.is_available(), "the warm-start pipeline runs on the GPU"
     dev = torch.device("cuda", torch.cuda.current_device())
 
     # 1. training data from the batched solver
-    b, steps, conv, wall, _ = converge(solids, max_steps=args.max_steps)
-    log(f"training data: {int(conv.sum())}/{len(conv)} converged, steps {steps.min()}-{steps.max()}, {wall:.1f}s")
-    nu = b.effective_viscosity()
-    k_train = b.permeability()
-    fields = [b.sample_fields(s) for s in range(b.n_samples)]
-    del b
-    torch.cuda.empty_cache()
-    inputs = [sg.make_inputs(s, device=dev) for s in solids]
-    targets = [sg.make_targets(r, v, s, FX, nu) for (r, v), s in zip(fields, solids)]
-    del fields
-    result["data"] = {"samples": len(solids), "converged": int(conv.sum()), "steps_max": int(steps.max()),
-                      "solver_wall_s": round(wall, 2), "prep_wall_s": round(time.time() - t0, 1)}
+    inputs, targets, k_train, nu, info = training_data(solids, args.max_steps)
+    info["prep_wall_s"] = round(time.time() - t0, 1)
+    result["data"] = info
 
     # 2. training
     net = sg.UNet3D(dev, seed=0)
@@ -225,88 +329,17 @@ def main(argv=None):
         del sampler
     del inputs, targets
     torch.cuda.empty_cache()
-    # 3. velocity scale: the training samples' permeability against the
-    # surrogate's own (from its warm start at t = 0), median ratio
-    scale = 1.0
-    if args.calibrate:
-        k_tr_pred = initial_permeability(net, solids, nu)
-        scale = float(np.median(k_train / k_tr_pred))
-        log(f"calibration: velocity scale {scale:.4f} (training k ratio {np.min(k_train / k_tr_pred):.3f}"
-            f"-{np.max(k_train / k_tr_pred):.3f})")
+    # 3. velocity scale fitted on the training samples
+    scale = velocity_scale(net, solids, k_train, nu) if args.calibrate else 1.0
     result["train"]["velocity_scale"] = scale
     if args.save_model:
         torch.save(net.state_dict(), args.save_model)
 
-    # 3. evaluation on held-out packs
-    ev = [sphere_pack((args.eval_size,) * 3, porosity=0.3, rmin=4.0, rmax=12.0, seed=args.eval_seed + i,
-                      periodic=(True, True, True)) for i in range(args.eval_samples)]
+    # 4. evaluation on held-out packs
+    ev = eval_packs(args.eval_samples, args.eval_size, args.eval_seed)
     log(f"evaluation: {len(ev)} packs of {args.eval_size}^3")
-    bc, steps_c, conv_c, wall_c, hist_c = converge(ev, max_steps=args.max_steps)
-    k_cold = bc.permeability()
-    ref = [bc.sample_fields(s) for s in range(bc.n_samples)]
-    del bc
-    torch.cuda.empty_cache()
-    log(f"cold: {int(conv_c.sum())} converged, steps median {np.median(steps_c):.0f}, {wall_c:.1f}s")
-
-    # the geometry goes to the device once, as the solver's own copy does, before the clock
-    ev_dev = [torch.from_numpy(np.ascontiguousarray(s)).to(dev) for s in ev]
-    torch.cuda.synchronize()
-    t_inf = time.perf_counter()
-    preds = sg.predict(net, ev_dev, FX, nu, density=args.density)
-    if scale != 1.0:
-        preds = [(r, v * scale) for r, v in preds]
-    torch.cuda.synchronize()
-    inf_s = time.perf_counter() - t_inf
-    fallback = np.array([not (torch.isfinite(r).all() and torch.isfinite(v).all()) for r, v in preds])
-    preds = [(torch.ones_like(r), torch.zeros_like(v)) if fb else (r, v) for (r, v), fb in zip(preds, fallback)]
-    # the surrogate's error, judged by the converged cold-start fields
-    v_err, rho_err = [], []
-    for (r, v), (rr, vr), s in zip(preds, ref, ev):
-        fl = s == 0
-        vp = v.cpu().numpy()
-        v_err.append(float(np.linalg.norm(vp[fl] - vr[fl]) / np.linalg.norm(vr[fl])))
-        # density deviation from the sample's fluid mean, against the cold start's
-        dr, dp = rr[fl] - rr[fl].mean(), r.cpu().numpy()[fl] - 1.0
-        rho_err.append(float(np.linalg.norm(dp - dr) / np.linalg.norm(dr)))
-    try:
-        bw, steps_w, conv_w, wall_w, hist_w = converge(ev, max_steps=args.max_steps, init_fields=preds)
-        k_warm = bw.permeability()
-        del bw
-    except FloatingPointError as e:  # the warm run diverged: every sample falls back to the cold start
-        log(f"warm start diverged ({e}); falling back to cold start")
-        fallback[:] = True
-        steps_w, conv_w, wall_w, hist_w, k_warm = steps_c, conv_c, wall_c, hist_c, k_cold
-    fallback |= ~conv_w
-    k_warm = np.where(fallback, k_cold, k_warm)
-    k_pred = hist_w[0][2]
-    signed = (k_pred - k_cold) / np.abs(k_cold)
-    s1_c, w1_c = within(hist_c, k_cold)
-    s1_w, w1_w = within(hist_w, k_cold)
-    err = np.abs(k_warm - k_cold) / np.abs(k_cold)
-    perr = np.abs(k_pred - k_cold) / np.abs(k_cold)
-    result["eval"] = {
-        "samples": len(ev), "size": args.eval_size, "fallbacks": int(fallback.sum()),
-        "surrogate": {"v_rel_l2_median": float(np.median(v_err)), "v_rel_l2_max": float(np.max(v_err)),
-                      "rho_dev_rel_l2_median": float(np.median(rho_err)),
-                      "k_rel_err_median": float(np.median(perr)), "k_rel_err_max": float(np.max(perr)),
-                      "k_signed_err_median": float(np.median(signed)), "velocity_scale": scale,
-                      "inference_s": round(inf_s, 3)},
-        "cold": {"steps_median": float(np.median(steps_c)), "steps_max": int(steps_c.max()),
-                 "wall_s": round(wall_c, 2), "converged": int(conv_c.sum()),
-                 "steps_to_1pct_median": float(np.median(s1_c)), "steps_to_1pct_max": int(s1_c.max()),
-                 "wall_to_1pct_s": round(float(np.nanmax(w1_c)), 3)},
-        "warm": {"steps_median": float(np.median(steps_w)), "steps_max": int(steps_w.max()),
-                 "wall_s": round(wall_w + inf_s, 2), "converged": int(conv_w.sum()),
-                 "steps_to_1pct_median": float(np.median(s1_w)), "steps_to_1pct_max": int(s1_w.max()),
-                 "wall_to_1pct_s": round(float(np.nanmax(w1_w)) + inf_s, 3)},
-        "permeability_rel_err_warm_vs_cold": {"median": float(np.median(err)), "max": float(err.max())},
-        "speedup_to_steady": round(wall_c / (wall_w + inf_s), 3),
-        "speedup_to_1pct": round(float(np.nanmax(w1_c)) / (float(np.nanmax(w1_w)) + inf_s), 3),
-        # the same for the median sample: the batch's wall-clock when that
-        # sample came within 1 % (cold), against inference plus the same (warm)
-        "speedup_to_1pct_median_sample": round(float(np.nanmedian(w1_c)) / (float(np.nanmedian(w1_w)) + inf_s), 3),
-    }
-    log(json.dumps(result["eval"]))
+    result["eval"] = evaluate(net, ev, nu, scale, density=args.density, max_steps=args.max_steps)
+    log(json.dumps({k: v for k, v in result["eval"].items() if k != "per_sample"}))
     line = json.dumps(result)
     if args.out:
         with open(args.out, "w") as fh:

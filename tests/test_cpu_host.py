@@ -378,6 +378,48 @@ def test_sphere_pack_properties():
     assert (a[:, 0, :] & a[:, -1, :]).any() and (a[:, :, 0] & a[:, :, -1]).any()
 
 
+def sphere_pack_one_at_a_time(shape, porosity, rmin, rmax, seed, periodic):
+    """The pack's definition, a sphere at a time: draw the centre (3 uniforms
+    times the extents) and the radius, mark the cells within r (wrapped along
+    periodic axes, clipped along the others), stop once the solid count
+    reaches (1 - porosity) of the box."""
+    n = np.array(shape)
+    rng = np.random.default_rng(seed)
+    solid = np.zeros(shape, np.uint8)
+    target = (1.0 - porosity) * solid.size
+    while solid.sum(dtype=np.int64) < target:
+        c = rng.uniform(0.0, 1.0, 3) * n
+        r = rng.uniform(rmin, rmax)
+        idx, dist2 = [], 0.0
+        for a in range(3):
+            cells = np.arange(int(np.floor(c[a] - r)), int(np.ceil(c[a] + r)) + 1)
+            d = cells - c[a]
+            if periodic[a]:
+                cells = np.mod(cells, n[a])
+            else:
+                keep = (cells >= 0) & (cells < n[a])
+                cells, d = cells[keep], d[keep]
+            shp = [1, 1, 1]
+            shp[a] = -1
+            idx.append(cells.reshape(shp))
+            dist2 = dist2 + (d * d).reshape(shp)
+        ii, jj, kk = np.broadcast_arrays(*idx)
+        inside = dist2 <= r * r
+        solid[ii[inside], jj[inside], kk[inside]] = 1
+    return solid
+
+
+@pytest.mark.parametrize("case", [((32, 32, 32), 0.35, 3.0, 6.0, 7, (False, True, True)),
+                                  ((48, 40, 40), 0.4, 3, 6, 1, (False, False, False)),
+                                  ((10, 12, 9), 0.3, 3, 7, 4, (True, True, True)),
+                                  ((5, 6, 40), 0.5, 2, 9, 3, (True, False, True)),
+                                  ((71, 71, 71), 0.37, 4.0, 12.0, 1000, (True, True, True))])
+def test_sphere_pack_equals_one_sphere_at_a_time(case):
+    """The chunked generator gives the cells of the sequential definition."""
+    from taichi_lbm3d_amd.geometry import sphere_pack
+    np.testing.assert_array_equal(sphere_pack(*case), sphere_pack_one_at_a_time(*case))
+
+
 def test_cube_channel_properties():
     from taichi_lbm3d_amd.geometry import cube_channel
     g = cube_channel((40, 32, 32), fraction=0.05, seed=0)
