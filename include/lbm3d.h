@@ -39,7 +39,7 @@
 extern "C" {
 #endif
 
-#define LBM3D_ABI_VERSION 11
+#define LBM3D_ABI_VERSION 12
 
 /* storage types for populations; F32, F16 and BF16 compute in FP32, F64 in FP64 */
 #define LBM3D_F32 0
@@ -269,6 +269,14 @@ size_t lbm3d_reduce_workspace_bytes(int32_t n_cells);
 int lbm3d_reduce(void *stream, const lbm3d_params *p, const int32_t *cells, const float *rho,
                  const float *v, int32_t i_lo, int32_t i_hi, double *out, void *workspace,
                  size_t workspace_bytes);
+
+/* The same reductions over FP64 fields (the lbm3d_macros output of FP64
+ * storage, LBM3D_F64), so that FP64 runs are not rounded to FP32 before they
+ * are summed. */
+int lbm3d_reduce64(void *stream, const lbm3d_params *p, const int32_t *cells, const double *rho,
+                   const double *v, int32_t i_lo, int32_t i_hi, double *out, void *workspace,
+                   size_t workspace_bytes);
+
 
 
 /* ---- batches: many samples of uneven size in one launch per step ----------

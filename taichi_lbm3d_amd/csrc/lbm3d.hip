@@ -1448,7 +1448,8 @@ __device__ __forceinline__ void block_fold(double acc[N], double *__restrict__ o
 }
 
 // quantities 0-5: sum rho, sum v (3), max |v|, non-finite cells
-__device__ __forceinline__ bool accumulate(double *acc, float r, float a, float b, float c) {
+template <typename R>
+__device__ __forceinline__ bool accumulate(double *acc, R r, R a, R b, R c) {
   const bool fin = isfinite(r) && isfinite(a) && isfinite(b) && isfinite(c);
   if (!fin) {
     acc[5] += 1.0;
@@ -1464,14 +1465,15 @@ __device__ __forceinline__ bool accumulate(double *acc, float r, float a, float 
 }
 
 // 6-8: sum vx, cells and sum rho over the slabs i_lo <= i <= i_hi; 9: 0
+// (FP32 fields, or the FP64 fields of FP64 storage)
+template <typename R>
 __global__ void __launch_bounds__(kBlock) k_reduce_partial(const KP kp, const int32_t *__restrict__ cells,
-                                                           const float *__restrict__ rho,
-                                                           const float *__restrict__ v, int32_t i_lo,
-                                                           int32_t i_hi, double *__restrict__ part) {
+                                                           const R *__restrict__ rho, const R *__restrict__ v,
+                                                           int32_t i_lo, int32_t i_hi, double *__restrict__ part) {
   const uint32_t N = (uint32_t)kp.p.n_cells;
   double acc[kRedGrid] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
   for (uint32_t x = blockIdx.x * kBlock + threadIdx.x; x < N; x += gridDim.x * kBlock) {
-    const float a = v[x], r = rho[x];
+    const R a = v[x], r = rho[x];
     if (!accumulate(acc, r, a, v[(size_t)N + x], v[2 * (size_t)N + x])) continue;
     const int i = (int)fdiv((uint32_t)cells[x], kp.geo.m_nyz, kp.geo.s_nyz);
     if (i >= i_lo && i <= i_hi) {
@@ -1827,6 +1829,28 @@ KP make_kp_batch(const lbm3d_params &p) {
   return k;
 }
 
+size_t reduce_workspace_bytes() { return (size_t)kRedBlocks * kRedGrid * sizeof(double); }
+
+// The two passes of lbm3d_reduce / lbm3d_reduce64 over fields of type R.
+template <typename R>
+int launch_reduce(void *stream, const lbm3d_params *p, const int32_t *cells, const R *rho, const R *v, int32_t i_lo,
+                  int32_t i_hi, double *out, void *workspace, size_t workspace_bytes, const char *what) {
+  int rc = check_params(p);
+  if (rc) return rc;
+  if ((rc = check_cells(p))) return rc;
+  if (!out || !workspace || (p->n_cells > 0 && (!cells || !rho || !v)))
+    return fail(LBM3D_EINVAL, "NULL argument to %s", what);
+  if (workspace_bytes < reduce_workspace_bytes()) return fail(LBM3D_EINVAL, "%s: workspace too small", what);
+  int blocks = (p->n_cells + kBlock - 1) / kBlock;
+  blocks = blocks < 1 ? 1 : (blocks > kRedBlocks ? kRedBlocks : blocks);
+  const KP kp = make_kp(*p);
+  hipStream_t s = (hipStream_t)stream;
+  k_reduce_partial<R><<<blocks, kBlock, 0, s>>>(kp, cells, rho, v, i_lo, i_hi, (double *)workspace);
+  if ((rc = check_launch(what))) return rc;
+  k_reduce_final<<<1, 64 * kRedGrid, 0, s>>>((const double *)workspace, blocks, out);
+  return check_launch(what);
+}
+
 }  // namespace
 
 extern "C" {
@@ -2131,28 +2155,19 @@ int lbm3d_scatter_dense(void *stream, const lbm3d_params *p, const int32_t *cell
 
 size_t lbm3d_reduce_workspace_bytes(int32_t n_cells) {
   (void)n_cells;
-  return (size_t)kRedBlocks * kRedGrid * sizeof(double);
+  return reduce_workspace_bytes();
 }
 
 int lbm3d_reduce(void *stream, const lbm3d_params *p, const int32_t *cells, const float *rho, const float *v,
                  int32_t i_lo, int32_t i_hi, double *out, void *workspace, size_t workspace_bytes) {
-  int rc = check_params(p);
-  if (rc) return rc;
-  if ((rc = check_cells(p))) return rc;
-  if (!out || !workspace || (p->n_cells > 0 && (!cells || !rho || !v)))
-    return fail(LBM3D_EINVAL, "%s", "NULL argument to lbm3d_reduce");
-  if (workspace_bytes < lbm3d_reduce_workspace_bytes(p->n_cells))
-    return fail(LBM3D_EINVAL, "%s", "reduction workspace too small");
-  int blocks = (p->n_cells + kBlock - 1) / kBlock;
-  blocks = blocks < 1 ? 1 : (blocks > kRedBlocks ? kRedBlocks : blocks);
-  const KP kp = make_kp(*p);
-  hipStream_t s = (hipStream_t)stream;
-  k_reduce_partial<<<blocks, kBlock, 0, s>>>(kp, cells, rho, v, i_lo, i_hi, (double *)workspace);
-  if ((rc = check_launch("lbm3d_reduce"))) return rc;
-  k_reduce_final<<<1, 64 * kRedGrid, 0, s>>>((const double *)workspace, blocks, out);
-  return check_launch("lbm3d_reduce/final");
+  return launch_reduce<float>(stream, p, cells, rho, v, i_lo, i_hi, out, workspace, workspace_bytes, "lbm3d_reduce");
 }
 
+int lbm3d_reduce64(void *stream, const lbm3d_params *p, const int32_t *cells, const double *rho, const double *v,
+                   int32_t i_lo, int32_t i_hi, double *out, void *workspace, size_t workspace_bytes) {
+  return launch_reduce<double>(stream, p, cells, rho, v, i_lo, i_hi, out, workspace, workspace_bytes,
+                               "lbm3d_reduce64");
+}
 
 // ---- batches ---------------------------------------------------------------
 int lbm3d_batch_layout(const lbm3d_params *p, lbm3d_sample *samples, int32_t n_samples, lbm3d_batch_sizes *sizes) {

@@ -537,14 +537,17 @@ class LB3D_Solver_Single_Phase:
         torch = _torch()
         p = self._params
         dev = self._torch_device()
-        rho_c, v_c = self.compact_macros()
+        # FP64 storage sums its FP64 fields; the others their FP32 fields
+        native = self.dtype == "f64"
+        rho_c, v_c = self.compact_macros(native=native)
         lib = _lib.lib()
         ws_bytes = lib.size("lbm3d_reduce_workspace_bytes", p.n_cells)
         ws = torch.empty(ws_bytes, dtype=torch.uint8, device=dev)
         out = torch.zeros(10, dtype=torch.float64, device=dev)
         with torch.cuda.device(dev):
-            lib.call("lbm3d_reduce", self._stream(), _ref(p), _ptr(self._dev["cells"]), _ptr(rho_c), _ptr(v_c),
-                     int(i_lo), int(self.nx - 1 if i_hi is None else i_hi), _ptr(out), _ptr(ws), ws_bytes)
+            lib.call("lbm3d_reduce64" if native else "lbm3d_reduce", self._stream(), _ref(p),
+                     _ptr(self._dev["cells"]), _ptr(rho_c), _ptr(v_c), int(i_lo),
+                     int(self.nx - 1 if i_hi is None else i_hi), _ptr(out), _ptr(ws), ws_bytes)
         o = out.cpu().numpy()
         return {"sum_rho": o[0], "sum_v": o[1:4].copy(), "max_v": o[4], "nonfinite": int(o[5]),
                 "slab_sum_vx": o[6], "slab_count": int(o[7]), "slab_sum_rho": o[8]}

@@ -37,37 +37,53 @@ def pressure_solver(solid, dtype="f32", sparse=True, wave=0):
 def test_porous_pressure_run_matches_oracle(gpu):
     """The committed 32^3 sphere pack with the benchmark's drive (sparse
     storage, pressure x faces 1.0005 / 1.0, y and z periodic, niu 0.1) for
-    4,000 steps: FP32 and FP64 storage against the FP64 oracle, and
-    permeability() against the pressure-drive Darcy formula evaluated in FP64
-    on the oracle's fields."""
+    4,000 steps against the FP64 oracle.  FP64 storage matches to round-off.
+    FP32 storage is held to 1e-4 in v and, more tellingly, to a twentieth of
+    the error of the reference's own FP32 formulation (the FP32 oracle, which
+    stores f unshifted as SP:31-33 does): 4e-3 on this run, where the small
+    pressure-driven velocities are differences of O(1) populations.
+    permeability() is checked against the pressure-drive Darcy formula
+    evaluated in FP64 on the oracle's fields."""
     solid = pack32()
     steps = 4000
-    o = COracle(32, 32, 32, sparse_storage=True, real="f64", threads=THREADS)
-    o.set_solid(solid)
-    o.niu = 0.1
-    o.set_bc_rho("x_left", 1.0005)
-    o.set_bc_rho("x_right", 1.0)
-    o.init_simulation()
-    t0 = time.time()
-    o.step(steps)
     fl = solid == 0
-    orho, ov = o.rho()[:32, :32, :32], o.v()[:32, :32, :32]
+    t0 = time.time()
+    ref = {}
+    for real in ("f64", "f32"):
+        o = COracle(32, 32, 32, sparse_storage=True, real=real, threads=THREADS)
+        o.set_solid(solid)
+        o.niu = 0.1
+        o.set_bc_rho("x_left", 1.0005)
+        o.set_bc_rho("x_right", 1.0)
+        o.init_simulation()
+        o.step(steps)
+        ref[real] = (o.rho()[:32, :32, :32], o.v()[:32, :32, :32])
+    orho, ov = ref["f64"]
+    ref32_v = rel_l2(ref["f32"][1][fl], ov[fl])
     # Darcy, pressure drive: k = nu_eff q (nx - 1) / (drho / 3), q the superficial
     # mean x velocity over the slabs 1 <= i <= nx - 2
     nu = (0.1 / 3.0 + 0.5 - 0.5) / 3.0
     q = (ov[..., 0] * fl)[1:31].sum() / (32 * 32 * 30)
     k_np = nu * q * 31 / (0.0005 / 3.0)
-    for dtype, tol_v, tol_k in (("f32", 1e-5, 1e-5), ("f64", 1e-11, 1e-11)):
+    assert k_np > 0
+    print(f"oracle FP64 + FP32 {time.time() - t0:.1f}s; FP32 oracle v rel-L2 against FP64 {ref32_v:.2e}")
+    for dtype, tol_rho, tol_v, tol_k in (("f32", 1e-6, min(1e-4, 0.05 * ref32_v), 2e-4),
+                                         ("f64", 1e-12, 1e-11, 1e-10)):
         s = pressure_solver(solid, dtype)
         s.run(steps)
         rho, v = s.rho.to_numpy()[:32, :32, :32], s.v.to_numpy()[:32, :32, :32]
+        if dtype == "f64":  # the FP64 fields themselves, not their FP32 copies
+            r64, v64 = s.compact_macros(native=True)
+            n = s.n_fluid
+            rho = rho.astype(np.float64)
+            v = v.astype(np.float64)
+            rho[fl] = r64[:n].cpu().numpy()
+            v[fl] = v64[: 3 * n].cpu().numpy().reshape(3, n).T
         er, ev = rel_l2(rho[fl], orho[fl]), rel_l2(v[fl], ov[fl])
         k = s.permeability()
-        print(f"{dtype}: rel-L2 rho {er:.2e} v {ev:.2e}, k {k:.6g} vs {k_np:.6g} "
-              f"({abs(k / k_np - 1):.1e}); oracle {time.time() - t0:.1f}s")
-        assert er < tol_v and ev < tol_v
+        print(f"{dtype}: rel-L2 rho {er:.2e} v {ev:.2e}, k {k:.9g} vs {k_np:.9g} ({abs(k / k_np - 1):.1e})")
+        assert er < tol_rho and ev < tol_v
         assert abs(k / k_np - 1) < tol_k
-    assert k_np > 0
 
 
 def slit(conv, niu, fx, dtype):
