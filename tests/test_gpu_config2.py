@@ -37,7 +37,9 @@ def pressure_solver(solid, dtype="f32", sparse=True, wave=0):
 def test_porous_pressure_run_matches_oracle(gpu):
     """The committed 32^3 sphere pack with the benchmark's drive (sparse
     storage, pressure x faces 1.0005 / 1.0, y and z periodic, niu 0.1) for
-    4,000 steps against the FP64 oracle.  FP64 storage matches to round-off.
+    4,000 steps against the FP64 oracle.  FP64 storage matches to round-off
+    (1e-11 relative in v: the velocities, ~1e-5, are differences of O(1)
+    populations, so FP64's 1e-16 grows to that level).
     FP32 storage is held to 1e-4 in v and, more tellingly, to a twentieth of
     the error of the reference's own FP32 formulation (the FP32 oracle, which
     stores f unshifted as SP:31-33 does): 4e-3 on this run, where the small
@@ -68,7 +70,7 @@ def test_porous_pressure_run_matches_oracle(gpu):
     assert k_np > 0
     print(f"oracle FP64 + FP32 {time.time() - t0:.1f}s; FP32 oracle v rel-L2 against FP64 {ref32_v:.2e}")
     for dtype, tol_rho, tol_v, tol_k in (("f32", 1e-6, min(1e-4, 0.05 * ref32_v), 2e-4),
-                                         ("f64", 1e-12, 1e-11, 1e-10)):
+                                         ("f64", 1e-12, 1e-10, 1e-10)):
         s = pressure_solver(solid, dtype)
         s.run(steps)
         rho, v = s.rho.to_numpy()[:32, :32, :32], s.v.to_numpy()[:32, :32, :32]
