@@ -277,6 +277,17 @@ int lbm3d_reduce64(void *stream, const lbm3d_params *p, const int32_t *cells, co
                    const double *v, int32_t i_lo, int32_t i_hi, double *out, void *workspace,
                    size_t workspace_bytes);
 
+/* lbm3d_reduce without a workspace argument: no counterpart in the reference.
+ * For callers that cannot allocate device memory (code under HIP graph
+ * capture, tight convergence loops, harnesses that forbid hipMalloc).  The
+ * per-block partials live in a fixed 80 KiB device array inside the library
+ * (one copy per GPU, held until the process ends); nothing is allocated.  The
+ * partials are fully rewritten by each call, so any n_cells that lbm3d_reduce
+ * accepts is accepted.  Same results as lbm3d_reduce, bit for bit.  Calls that
+ * overlap on one GPU (concurrent streams) share the array and race: serialise
+ * them. */
+int lbm3d_reduce_noalloc(void *stream, const lbm3d_params *p, const int32_t *cells, const float *rho,
+                         const float *v, int32_t i_lo, int32_t i_hi, double *out);
 
 
 /* ---- batches: many samples of uneven size in one launch per step ----------

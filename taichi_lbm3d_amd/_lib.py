@@ -130,6 +130,7 @@ SIGNATURES = {
     "lbm3d_reduce_workspace_bytes": (_sz, [_i32]),
     "lbm3d_reduce": (_int, [_vp, _pp, _vp, _vp, _vp, _i32, _i32, _vp, _vp, _sz]),
     "lbm3d_reduce64": (_int, [_vp, _pp, _vp, _vp, _vp, _i32, _i32, _vp, _vp, _sz]),
+    "lbm3d_reduce_noalloc": (_int, [_vp, _pp, _vp, _vp, _vp, _i32, _i32, _vp]),
     "lbm3d_batch_layout": (_int, [_pp, ctypes.POINTER(Sample), _i32, ctypes.POINTER(BatchSizes)]),
     "lbm3d_batch_desc": (_int, [_pp, ctypes.POINTER(Sample), _i32, _vp, _sz]),
     "lbm3d_batch_init": (_int, [_vp, _pp, _int, _vp, _vp, _i32, _vp, _vp, _vp, _vp]),

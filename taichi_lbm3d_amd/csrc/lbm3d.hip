@@ -1829,6 +1829,34 @@ KP make_kp_batch(const lbm3d_params &p) {
   return k;
 }
 
+// Scratch of lbm3d_reduce_noalloc: the per-block partials of the reduction
+// (kRedBlocks x kRedGrid doubles, 80 KiB), a file-scope device array, so the
+// entry point allocates nothing.  Each GPU holds its own copy at its own
+// address; host code must use the address hipGetSymbolAddress returns on that
+// device, never the array's name.  Calls that overlap on one GPU share it and race.
+static __device__ __align__(256) double scratch[kRedBlocks * kRedGrid];
+
+constexpr int kMaxDevices = 64;
+std::atomic<double *> g_scratch_ptr[kMaxDevices];
+
+// The current device's address of `scratch`, looked up once per device.
+int reduce_scratch(double **ptr) {
+  int dev = -1;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return fail(LBM3D_EHIP, "hipGetDevice: %s", hipGetErrorString(e));
+  if (dev < 0 || dev >= kMaxDevices) return fail(LBM3D_EINVAL, "%s", "device index beyond the scratch table");
+  double *p = g_scratch_ptr[dev].load(std::memory_order_acquire);
+  if (!p) {
+    void *addr = nullptr;
+    e = hipGetSymbolAddress(&addr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !addr) return fail(LBM3D_EHIP, "hipGetSymbolAddress(scratch): %s", hipGetErrorString(e));
+    p = (double *)addr;
+    g_scratch_ptr[dev].store(p, std::memory_order_release);
+  }
+  *ptr = p;
+  return LBM3D_OK;
+}
+
 size_t reduce_workspace_bytes() { return (size_t)kRedBlocks * kRedGrid * sizeof(double); }
 
 // The two passes of lbm3d_reduce / lbm3d_reduce64 over fields of type R.
@@ -2167,6 +2195,26 @@ int lbm3d_reduce64(void *stream, const lbm3d_params *p, const int32_t *cells, co
                    int32_t i_lo, int32_t i_hi, double *out, void *workspace, size_t workspace_bytes) {
   return launch_reduce<double>(stream, p, cells, rho, v, i_lo, i_hi, out, workspace, workspace_bytes,
                                "lbm3d_reduce64");
+}
+
+int lbm3d_reduce_noalloc(void *stream, const lbm3d_params *p, const int32_t *cells, const float *rho, const float *v,
+                         int32_t i_lo, int32_t i_hi, double *out) {
+  int rc = check_params(p);
+  if (rc) return rc;
+  if ((rc = check_cells(p))) return rc;
+  if (!out || (p->n_cells > 0 && (!cells || !rho || !v)))
+    return fail(LBM3D_EINVAL, "%s", "NULL argument to lbm3d_reduce_noalloc");
+  double *part = nullptr;
+  if ((rc = reduce_scratch(&part))) return rc;
+  int blocks = (p->n_cells + kBlock - 1) / kBlock;
+  blocks = blocks < 1 ? 1 : (blocks > kRedBlocks ? kRedBlocks : blocks);
+  const KP kp = make_kp(*p);
+  hipStream_t s = (hipStream_t)stream;
+  // every partial the final pass reads is written by this call's first pass
+  k_reduce_partial<float><<<blocks, kBlock, 0, s>>>(kp, cells, rho, v, i_lo, i_hi, part);
+  if ((rc = check_launch("lbm3d_reduce_noalloc"))) return rc;
+  k_reduce_final<<<1, 64 * kRedGrid, 0, s>>>(part, blocks, out);
+  return check_launch("lbm3d_reduce_noalloc/final");
 }
 
 // ---- batches ---------------------------------------------------------------
