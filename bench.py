@@ -8,7 +8,7 @@ niu = 0.1, FP32 populations.  A step is one fused collide+stream launch over
 the fluid list.  Prints one JSON line.
 
   python bench.py [--gpus 1] [--steps K] [--warmup W] [--dtype f32|f16|bf16|f64|f16sr|bf16sr]
-                  [--wave CHUNKS] [--no-steady] [--no-precision] [--no-cpu-baseline]
+                  [--no-steady] [--no-precision] [--no-cpu-baseline]
 
 With FP32 storage the line also carries `precision`: every 16-bit storage
 format on the same pack (MLUPS, steps and wall time to the steady criterion,
@@ -30,7 +30,6 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0           # MI355X HBM3E, GB/s
 BYTES_PER_FLUP = {"f64": 304, "f32": 152, "f16": 76, "bf16": 76, "f16sr": 76, "bf16sr": 76}  # 19 reads + 19 writes per fluid-cell update
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "step_traffic_pack256.json")
-WAVE_CHUNKS = 0  # default step order: one launch per step
 
 
 def log(msg):
@@ -45,7 +44,7 @@ def make_geometry(n, seed):
     return solid
 
 
-def make_solver(solid, dtype, wave=0):
+def make_solver(solid, dtype):
     from taichi_lbm3d_amd import LB3D_Solver_Single_Phase
     n = solid.shape
     s = LB3D_Solver_Single_Phase(*n, sparse_storage=True, dtype=dtype)
@@ -53,7 +52,6 @@ def make_solver(solid, dtype, wave=0):
     s.niu = 0.1
     s.set_bc_rho_x0(1.0005)
     s.set_bc_rho_x1(1.0)
-    s.wave_chunks = wave
     s.init_simulation()
     return s
 
@@ -119,9 +117,9 @@ def time_steps(s, steps, warmup):
     return start.elapsed_time(end)
 
 
-def run_steady(solid, dtype, max_steps, wave):
+def run_steady(solid, dtype, max_steps):
     import torch
-    s = make_solver(solid, dtype, wave)
+    s = make_solver(solid, dtype)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     steps, ok, hist = s.run_to_steady(tol=1e-6, interval=500, max_steps=max_steps)
@@ -134,21 +132,21 @@ def run_steady(solid, dtype, max_steps, wave):
             "permeability_gradient_lu2": kg, "mean_vx": hist[-1]}
 
 
-def precision_study(solid, f32_mlups, f32_steady, max_steps, wave):
+def precision_study(solid, f32_mlups, f32_steady, max_steps):
     """Each 16-bit storage format on the same pack: MLUPS (400 timed steps), its
     own run to the 1e-6 criterion, and the permeability against an FP64 run to
     the same criterion."""
     import torch
-    ref = run_steady(solid, "f64", max_steps, wave)
+    ref = run_steady(solid, "f64", max_steps)
     k64 = ref["permeability_lu2"]
     out = {"reference": {"dtype": "f64", **ref}}
     rows = {"f32": (f32_mlups, f32_steady)}
     for dt in ("f16", "bf16", "f16sr", "bf16sr"):
-        s = make_solver(solid, dt, wave)
+        s = make_solver(solid, dt)
         ms = time_steps(s, 400, 100)
         del s
         torch.cuda.empty_cache()
-        rows[dt] = (s_mlups(solid, ms, 400), run_steady(solid, dt, max_steps, wave))
+        rows[dt] = (s_mlups(solid, ms, 400), run_steady(solid, dt, max_steps))
         log(f"precision {dt}: {rows[dt][0]:.0f} MLUPS, {rows[dt][1]['steps']} steps, {rows[dt][1]['wall_s']}s")
     for dt, (mlups, st) in rows.items():
         out[dt] = {"mlups": round(mlups, 1), "vs_f32_mlups": round(mlups / f32_mlups, 3),
@@ -170,8 +168,6 @@ def main():
     ap.add_argument("--n", type=int, default=256)
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--dtype", default="f32", choices=["f32", "f16", "bf16", "f64", "f16sr", "bf16sr"])
-    ap.add_argument("--wave", type=int, default=WAVE_CHUNKS,
-                    help="chunks of the wavefront step order (0: one launch per step)")
     ap.add_argument("--no-steady", action="store_true")
     ap.add_argument("--no-precision", action="store_true")
     ap.add_argument("--max-steady-steps", type=int, default=50000)
@@ -186,7 +182,7 @@ def main():
     assert torch.cuda.is_available(), "bench.py needs a GPU"
     torch.cuda.set_device(0)
     solid = make_geometry(args.n, args.seed)
-    s = make_solver(solid, args.dtype, args.wave)
+    s = make_solver(solid, args.dtype)
     n_fluid = s.n_fluid
     log(f"fluid cells {n_fluid}")
 
@@ -204,20 +200,19 @@ def main():
     if os.path.exists(TRAFFIC_FILE) and args.n == 256 and args.dtype == "f32":
         with open(TRAFFIC_FILE) as fh:
             tf = json.load(fh)
-        if tf.get("wave_chunks", 0) == args.wave:
-            traffic = tf.get("bytes_per_launch")
+        traffic = tf.get("bytes_per_launch")
     log(f"timed {args.steps} steps: {ms:.1f} ms, {mlups:.0f} MLUPS, {achieved:.0f} GB/s algorithmic")
 
     steady = None
     if not args.no_steady:
-        steady = run_steady(solid, args.dtype, args.max_steady_steps, args.wave)
+        steady = run_steady(solid, args.dtype, args.max_steady_steps)
         steady["criterion"] = f"|d<vx>|/|<vx>| < 1e-6 over 500 steps, max {args.max_steady_steps}"
         log(f"steady: {steady['steps']} steps converged={steady['converged']} {steady['wall_s']:.2f}s "
             f"k={steady['permeability_lu2']:.5g}")
 
     precision = None
     if not args.no_precision and not args.no_steady and args.dtype == "f32":
-        precision = precision_study(solid, mlups, steady, args.max_steady_steps, args.wave)
+        precision = precision_study(solid, mlups, steady, args.max_steady_steps)
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -227,8 +222,7 @@ def main():
             cpu["steady_s_extrapolated"] = round(steady["steps"] * n_fluid / (cpu["value"] * 1e6), 1)
         log(f"cpu baseline: {cpu['value']:.1f} MLUPS on {cpu['cores']} cores")
 
-    schedule = (f"wavefront step pairs over {args.wave} chunks of planes" if args.wave
-                else "one launch per step")
+    schedule = "one launch per step"
     out = {
         "metric": "MLUPS + % HBM roofline, D3Q19 MRT 256^3 porous; time-to-steady permeability",
         "value": round(mlups, 1),

@@ -205,43 +205,6 @@ int lbm3d_run(void *stream, const lbm3d_params *p, int dtype, void *g, const int
               const lbm3d_row_word *rows, const uint32_t *face_bits, void *face_u, int64_t t0,
               int32_t nsteps);
 
-/* ---- wavefront schedule of step pairs --------------------------------------
- *
- * No reference counterpart: an order of the same per-cell updates as
- * lbm3d_run (SP:477-481 repeated) that keeps an odd step's gathers in the
- * last-level cache.  The fluid list is cut into m chunks of whole blocks
- * along x.  A pair of steps (t even, t+1) runs the even step of chunks 0 and
- * 1, then for c = 1 .. m-2 the even step of chunk c+1 followed by the odd
- * step of chunk c, and last the odd steps of chunks m-1 and 0 (whose
- * neighbours wrap across the periodic seam, SP:247-257).  An odd update reads
- * the slots that the even updates of its neighbour planes wrote one or two
- * launches before.  Results are bit-identical to lbm3d_run. */
-#define LBM3D_WAVE_MAX_CHUNKS 64
-
-typedef struct lbm3d_wave_plan {
-  int32_t n_chunks;   /* m, 3 .. LBM3D_WAVE_MAX_CHUNKS */
-  int32_t n_cells;    /* the fluid-cell count the plan was built for */
-  int32_t block_begin[LBM3D_WAVE_MAX_CHUNKS + 1];  /* chunk c = blocks [block_begin[c], block_begin[c+1])
-                                                    * of 256 cells; block_begin[m] = ceil(n_cells / 256) */
-} lbm3d_wave_plan;
-
-/* Host only.  plane_cells[nx] (host) = fluid cells of each plane i of the
- * list the plan is for.  Cuts the list into n_chunks chunks of about equal
- * size, or fewer (down to 3) until every middle chunk lies off the planes 0
- * and nx-1 and every neighbour plane of its cells lies in the chunks up to
- * the next one, the condition under which the order above is exact.
- * LBM3D_EINVAL when no such cut exists (too few planes). */
-int lbm3d_wave_plan_build(const lbm3d_params *p, const int64_t *plane_cells, int32_t n_chunks,
-                          lbm3d_wave_plan *plan);
-
-/* lbm3d_run in the wavefront order: every step pair starting at an even t
- * runs as above; a leading odd step or a trailing even step runs as one
- * launch.  `plan` must come from lbm3d_wave_plan_build for the same cell list
- * (its n_cells and block count are checked). */
-int lbm3d_run_wave(void *stream, const lbm3d_params *p, int dtype, void *g, const int32_t *cells,
-                   const lbm3d_row_word *rows, const uint32_t *face_bits, void *face_u, int64_t t0,
-                   int32_t nsteps, const lbm3d_wave_plan *plan);
-
 /* Macroscopic fields after t steps, what SP's streaming3 leaves in rho and v
  * (SP:376-388): rho[n_cells] and v (SoA [3][n_cells]), FP32 (FP64 for
  * LBM3D_F64).  lbm3d_scatter_dense and lbm3d_reduce take the FP32 form. */

@@ -89,13 +89,6 @@ class Launch(ctypes.Structure):
                 ("reserved", ctypes.c_int32)]
 
 
-WAVE_MAX_CHUNKS = 64  # LBM3D_WAVE_MAX_CHUNKS
-
-
-class WavePlan(ctypes.Structure):
-    """Mirror of struct lbm3d_wave_plan."""
-    _fields_ = [("n_chunks", ctypes.c_int32), ("n_cells", ctypes.c_int32),
-                ("block_begin", ctypes.c_int32 * (WAVE_MAX_CHUNKS + 1))]
 
 
 BATCH_ALIGN = 512  # LBM3D_BATCH_ALIGN
@@ -123,8 +116,6 @@ SIGNATURES = {
     "lbm3d_init": (_int, [_vp, _pp, _int, _vp, _vp, _vp, _vp, _vp]),
     "lbm3d_step": (_int, [_vp, _pp, _int, _vp, _vp, _vp, _vp, _vp, _i64]),
     "lbm3d_run": (_int, [_vp, _pp, _int, _vp, _vp, _vp, _vp, _vp, _i64, _i32]),
-    "lbm3d_wave_plan_build": (_int, [_pp, _vp, _i32, ctypes.POINTER(WavePlan)]),
-    "lbm3d_run_wave": (_int, [_vp, _pp, _int, _vp, _vp, _vp, _vp, _vp, _i64, _i32, ctypes.POINTER(WavePlan)]),
     "lbm3d_macros": (_int, [_vp, _pp, _int, _vp, _vp, _vp, _vp, _vp, _i64, _vp, _vp]),
     "lbm3d_scatter_dense": (_int, [_vp, _pp, _vp, _vp, _vp, _i64, _i32, _i32, _i32, _vp, _vp]),
     "lbm3d_reduce_workspace_bytes": (_sz, [_i32]),

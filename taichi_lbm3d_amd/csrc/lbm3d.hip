@@ -1027,7 +1027,7 @@ __device__ __forceinline__ void step_tile(const KP &kp, T *__restrict__ g, const
 }
 
 // One time step over the blocks lb_begin .. lb_begin + nb - 1 of the fluid
-// list (the whole list, or one chunk of the wavefront schedule).
+// list.
 template <typename T, bool ODD, bool FORCED>
 __global__ void __launch_bounds__(kBlock, LBM3D_STEP_MINB)
     k_step(const KP kp, T *__restrict__ g, const int32_t *__restrict__ cells, const uint4 *__restrict__ rows,
@@ -1624,46 +1624,6 @@ int launch_step(hipStream_t s, const KP &kp, void *g, const int32_t *cells, cons
   return check_launch("lbm3d_step");
 }
 
-// ---- wavefront schedule of step pairs ----------------------------------------
-// plane of cell id c given the prefix sums start[0..nx] of fluid cells per plane
-int32_t plane_of(const int64_t *start, int32_t nx, int64_t c) {
-  return (int32_t)(std::upper_bound(start, start + nx + 1, c) - start) - 1;
-}
-
-// A plan is safe when every middle chunk c (1 .. m-2) lies off the periodic
-// seam (planes 0 and nx-1, whose neighbours wrap to the far end of the list)
-// and every neighbour plane of its cells lies in chunks 0 .. c+1: then the odd
-// update of chunk c, run right after the even update of chunk c+1, reads only
-// even-updated slots and writes none that a later even update still reads.
-bool wave_plan_safe(const int64_t *start, int32_t nx, int64_t n, const int32_t *B, int32_t m) {
-  for (int32_t c = 1; c + 1 < m; ++c) {
-    const int64_t first = (int64_t)B[c] * kBlock, last = std::min<int64_t>((int64_t)B[c + 1] * kBlock, n) - 1;
-    if (first > last) return false;
-    const int32_t pf = plane_of(start, nx, first), pl = plane_of(start, nx, last);
-    if (pf < 1 || pl > nx - 2) return false;
-    if (start[pl + 2] > std::min<int64_t>((int64_t)B[c + 2] * kBlock, n)) return false;
-  }
-  return true;
-}
-
-// Even step t over chunks, each chunk's odd step t+1 right after the even step
-// of the chunk after it; the seam chunks 0 and m-1 take their odd step last.
-template <typename T>
-int launch_wave_pair(hipStream_t s, const KP &kp, void *g, const int32_t *cells, const uint4 *rows,
-                     const uint32_t *face_bits, void *face_u, int64_t t, const lbm3d_wave_plan &pl) {
-  const int32_t m = pl.n_chunks;
-  const int32_t *B = pl.block_begin;
-  auto run = [&](int64_t tt, int32_t c) {
-    return launch_step<T>(s, kp, g, cells, rows, face_bits, face_u, tt, (uint32_t)B[c], (uint32_t)(B[c + 1] - B[c]));
-  };
-  int rc;
-  if ((rc = run(t, 0)) || (rc = run(t, 1))) return rc;
-  for (int32_t c = 1; c + 1 < m; ++c)
-    if ((rc = run(t, c + 1)) || (rc = run(t + 1, c))) return rc;
-  if ((rc = run(t + 1, m - 1)) || (rc = run(t + 1, 0))) return rc;
-  return LBM3D_OK;
-}
-
 template <typename T>
 int launch_macros(hipStream_t s, const KP &kp, const void *g, const int32_t *cells, const uint4 *rows,
                   const uint32_t *face_bits, const void *face_u, int64_t t, void *rho, void *v) {
@@ -2056,91 +2016,6 @@ int lbm3d_run(void *stream, const lbm3d_params *p, int dtype, void *g, const int
       default: rc = launch_step<__hip_bfloat16>(s, kp, g, cells, r, face_bits, face_u, t0 + k); break;
     }
     if (rc) return rc;
-  }
-  return LBM3D_OK;
-}
-
-int lbm3d_wave_plan_build(const lbm3d_params *p, const int64_t *plane_cells, int32_t n_chunks,
-                          lbm3d_wave_plan *plan) {
-  int rc = check_params(p);
-  if (rc) return rc;
-  if ((rc = check_cells(p))) return rc;
-  if (!plane_cells || !plan) return fail(LBM3D_EINVAL, "%s", "NULL argument to lbm3d_wave_plan_build");
-  if (n_chunks < 3 || n_chunks > LBM3D_WAVE_MAX_CHUNKS)
-    return fail(LBM3D_EINVAL, "%s", "n_chunks must be in 3 .. LBM3D_WAVE_MAX_CHUNKS");
-  const int32_t nx = p->nx;
-  int64_t *start = new int64_t[(size_t)nx + 1];
-  start[0] = 0;
-  bool ok = true;
-  for (int32_t i = 0; i < nx; ++i) {
-    if (plane_cells[i] < 0 || plane_cells[i] > (int64_t)p->ny * p->nz) ok = false;
-    start[i + 1] = start[i] + plane_cells[i];
-  }
-  if (!ok || start[nx] != p->n_cells) {
-    delete[] start;
-    return fail(LBM3D_EINVAL, "%s", "plane_cells does not count n_cells fluid cells by plane");
-  }
-  const int64_t nb = step_blocks(*p);
-  memset(plan, 0, sizeof(*plan));
-  int32_t B[LBM3D_WAVE_MAX_CHUNKS + 1];
-  for (int32_t m = n_chunks; m >= 3; --m) {
-    if (nb < m) continue;
-    bool inc = true;
-    for (int32_t c = 0; c <= m; ++c) B[c] = (int32_t)((int64_t)c * nb / m);
-    for (int32_t c = 0; c < m; ++c) inc = inc && B[c + 1] > B[c];
-    if (inc && wave_plan_safe(start, nx, p->n_cells, B, m)) {
-      plan->n_chunks = m;
-      plan->n_cells = p->n_cells;
-      memcpy(plan->block_begin, B, sizeof(int32_t) * (size_t)(m + 1));
-      delete[] start;
-      return LBM3D_OK;
-    }
-  }
-  delete[] start;
-  return fail(LBM3D_EINVAL, "%s", "no safe wavefront plan: the grid has too few planes for 3 chunks");
-}
-
-int lbm3d_run_wave(void *stream, const lbm3d_params *p, int dtype, void *g, const int32_t *cells,
-                   const lbm3d_row_word *rows, const uint32_t *face_bits, void *face_u, int64_t t0,
-                   int32_t nsteps, const lbm3d_wave_plan *plan) {
-  int rc = solver_args(p, dtype, g, cells, rows, face_bits);
-  if (rc) return rc;
-  if (t0 < 0 || nsteps < 0) return fail(LBM3D_EINVAL, "%s", "negative step index or count");
-  if (!plan) return fail(LBM3D_EINVAL, "%s", "wave plan is NULL");
-  const int32_t m = plan->n_chunks;
-  if (plan->n_cells != p->n_cells || m < 3 || m > LBM3D_WAVE_MAX_CHUNKS || plan->block_begin[0] != 0 ||
-      plan->block_begin[m] != (int32_t)step_blocks(*p))
-    return fail(LBM3D_EINVAL, "%s", "wave plan was not built for this cell list");
-  for (int32_t c = 0; c < m; ++c)
-    if (plan->block_begin[c + 1] <= plan->block_begin[c])
-      return fail(LBM3D_EINVAL, "%s", "wave plan chunks are not increasing");
-  const KP kp = make_kp(*p);
-  hipStream_t s = (hipStream_t)stream;
-  const uint4 *r = (const uint4 *)rows;
-  int64_t t = t0;
-  const int64_t t_end = t0 + nsteps;
-  while (t < t_end) {
-    const bool pair = !(t & 1) && t + 2 <= t_end;
-    switch (dtype) {
-      case LBM3D_F32:
-        rc = pair ? launch_wave_pair<float>(s, kp, g, cells, r, face_bits, face_u, t, *plan)
-                  : launch_step<float>(s, kp, g, cells, r, face_bits, face_u, t);
-        break;
-      case LBM3D_F16:
-        rc = pair ? launch_wave_pair<__half>(s, kp, g, cells, r, face_bits, face_u, t, *plan)
-                  : launch_step<__half>(s, kp, g, cells, r, face_bits, face_u, t);
-        break;
-      case LBM3D_F64:
-        rc = pair ? launch_wave_pair<double>(s, kp, g, cells, r, face_bits, face_u, t, *plan)
-                  : launch_step<double>(s, kp, g, cells, r, face_bits, face_u, t);
-        break;
-      default:
-        rc = pair ? launch_wave_pair<__hip_bfloat16>(s, kp, g, cells, r, face_bits, face_u, t, *plan)
-                  : launch_step<__hip_bfloat16>(s, kp, g, cells, r, face_bits, face_u, t);
-        break;
-    }
-    if (rc) return rc;
-    t += pair ? 2 : 1;
   }
   return LBM3D_OK;
 }

@@ -157,10 +157,6 @@ class LB3D_Solver_Single_Phase:
         # odd steps locate their cells through tile descriptors (LBM3D_FLAG_TILES);
         # results are identical either way
         self.use_tiles = True
-        # wavefront schedule of step pairs (lbm3d_run_wave): number of chunks
-        # along x, or 0 for one launch per step; results are identical either way
-        self.wave_chunks = 0
-        self._wave_plan = None
         self._initialised = False
         self._compiled = False  # parameters frozen by the first step() (SP kernels compile then)
         self._t = 0
@@ -330,13 +326,6 @@ class LB3D_Solver_Single_Phase:
             lib.call("lbm3d_build_face_bits", s, _ref(p), _ptr(solid), _ptr(cells), _ptr(face_bits))
             torch.cuda.current_stream(dev).synchronize()
         del d2c, ws, solid
-        self._wave_plan = None
-        if self.wave_chunks:
-            plan = _lib.WavePlan()
-            planes = np.ascontiguousarray((self.solid._data == 0).sum(axis=(1, 2)), dtype=np.int64)
-            lib.call("lbm3d_wave_plan_build", _ref(p), planes.ctypes.data_as(_lib.ctypes.c_void_p),
-                     int(self.wave_chunks), _ref(plan))
-            self._wave_plan = plan
         self._dev.update(cells=cells, rows=rows, face_bits=face_bits)
         self._modes_built = self._bc_modes()
 
@@ -434,10 +423,7 @@ class LB3D_Solver_Single_Phase:
         args = (self._stream(), _ref(p), _lib.DTYPES[self.dtype], _ptr(d["g"]), _ptr(d["cells"]), _ptr(d["rows"]),
                 _ptr(d["face_bits"]), _ptr(d["face_u"]), self._t, int(nsteps))
         with torch.cuda.device(self._torch_device()):
-            if self._wave_plan is not None:
-                _lib.lib().call("lbm3d_run_wave", *args, _ref(self._wave_plan))
-            else:
-                _lib.lib().call("lbm3d_run", *args)
+            _lib.lib().call("lbm3d_run", *args)
         self._t += int(nsteps)
         self._macro_cache = None
 

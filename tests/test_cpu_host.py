@@ -188,9 +188,8 @@ def test_abi_rejects_bad_arguments_without_gpu():
 def test_batch_struct_layouts(tmp_path):
     """ctypes mirrors of lbm3d_sample, lbm3d_batch_sizes and lbm3d_launch have
     the C layout."""
-    from taichi_lbm3d_amd._lib import BatchSizes, Launch, Sample, WavePlan
-    structs = ((Sample, "lbm3d_sample"), (BatchSizes, "lbm3d_batch_sizes"), (Launch, "lbm3d_launch"),
-               (WavePlan, "lbm3d_wave_plan"))
+    from taichi_lbm3d_amd._lib import BatchSizes, Launch, Sample
+    structs = ((Sample, "lbm3d_sample"), (BatchSizes, "lbm3d_batch_sizes"), (Launch, "lbm3d_launch"))
     checks = []
     for cls, cname in structs:
         checks.append(f'printf("{cname}.size %zu\\n", sizeof({cname}));')
@@ -256,70 +255,6 @@ def test_batch_layout_host_only():
     shape[0].block, shape[1].block = 96, 256
     rc = lib.cdll.lbm3d_batch_step(None, ctypes.byref(p), 0, vp, vp, 4, vp, vp, vp, vp, None, 0, shape)
     assert rc == -1 and "block" in lib.last_error()
-
-
-def _wave_plan(solid, n_chunks):
-    from taichi_lbm3d_amd import _lib
-    lib = _lib.lib()
-    p = _lib.Params()
-    p.nx, p.ny, p.nz = solid.shape
-    p.n_cells = int((solid == 0).sum())
-    planes = np.ascontiguousarray((solid == 0).sum(axis=(1, 2)), dtype=np.int64)
-    plan = _lib.WavePlan()
-    rc = lib.cdll.lbm3d_wave_plan_build(ctypes.byref(p), ctypes.c_void_p(planes.ctypes.data), n_chunks,
-                                        ctypes.byref(plan))
-    return rc, plan
-
-
-@pytest.mark.parametrize("seed", range(8))
-def test_wave_plan_orders_every_odd_update_after_its_neighbours(seed):
-    """Replays the wavefront launch order of lbm3d_run_wave on the brute-force
-    neighbour table: when a chunk's odd update runs, every fluid neighbour of
-    each of its cells (whose slots it reads and writes, SP:259-269) has had its
-    even update, so the order is exact.  Geometries with thin and empty planes."""
-    rng = np.random.default_rng(seed)
-    nx, ny, nz = int(rng.integers(24, 60)), int(rng.integers(6, 14)), int(rng.integers(6, 40))
-    solid = (rng.random((nx, ny, nz)) < rng.uniform(0.2, 0.7)).astype(np.int8)
-    solid[rng.integers(0, nx, size=3)] = 1                      # empty planes
-    solid[rng.integers(0, nx, size=2), : ny // 2] = 1           # thin planes
-    cells, d2c = oc.compact(solid)
-    nbr = oc.neighbors(solid, cells, d2c)
-    n = cells.size
-    for want in (3, 5, 8, 16):
-        rc, plan = _wave_plan(solid, want)
-        if rc != 0:
-            continue
-        m = plan.n_chunks
-        assert 3 <= m <= want
-        B = [plan.block_begin[c] * 256 for c in range(m + 1)]
-        assert B[0] == 0 and B[m] >= n > B[m - 1]
-        even = np.zeros(n, bool)
-
-        def run(odd, c):
-            lo, hi = B[c], min(B[c + 1], n)
-            if odd:
-                nb = nbr[:, lo:hi]
-                assert even[nb[nb >= 0]].all(), (seed, want, c)
-            else:
-                even[lo:hi] = True
-
-        run(False, 0)
-        run(False, 1)
-        for c in range(1, m - 1):
-            run(False, c + 1)
-            run(True, c)
-        run(True, m - 1)
-        run(True, 0)
-        assert even.all()
-
-
-def test_wave_plan_rejects_grids_too_thin():
-    from taichi_lbm3d_amd import _lib
-    solid = np.zeros((3, 40, 40), np.int8)   # 3 planes: no middle chunk can avoid the seam
-    rc, _ = _wave_plan(solid, 8)
-    assert rc == -1 and "wavefront" in _lib.lib().last_error()
-    rc, plan = _wave_plan(np.zeros((64, 24, 24), np.int8), 8)
-    assert rc == 0 and plan.n_chunks == 8
 
 
 # ---- host logic ---------------------------------------------------------------

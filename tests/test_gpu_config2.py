@@ -1,6 +1,6 @@
 """Pressure-driven porous flow (the benchmark's physics), the permeability and
-steady-state helpers, Taichi's parameter freezing and the wavefront step
-order, on the GPU against the FP64 oracle and closed forms."""
+steady-state helpers and Taichi's parameter freezing, on the GPU against the
+FP64 oracle and closed forms."""
 import os
 import time
 
@@ -22,14 +22,13 @@ def pack32():
     return np.unpackbits(g["bits"])[: 32 ** 3].reshape(32, 32, 32).astype(np.int8)
 
 
-def pressure_solver(solid, dtype="f32", sparse=True, wave=0):
+def pressure_solver(solid, dtype="f32", sparse=True):
     from taichi_lbm3d_amd import LB3D_Solver_Single_Phase
     s = LB3D_Solver_Single_Phase(*solid.shape, sparse_storage=sparse, dtype=dtype)
     s.solid.from_numpy(solid)
     s.niu = 0.1
     s.set_bc_rho_x0(1.0005)
     s.set_bc_rho_x1(1.0)
-    s.wave_chunks = wave
     s.init_simulation()
     return s
 
@@ -193,7 +192,7 @@ def test_pack256_pressure_faces_and_slab_flux(gpu):
     import torch
     from taichi_lbm3d_amd.geometry import sphere_pack
     solid = sphere_pack((256,) * 3, porosity=0.35, rmin=6.0, rmax=14.0, seed=0, periodic=(False, True, True))
-    s = pressure_solver(solid, "f32", wave=16)
+    s = pressure_solver(solid, "f32")
     cells = s._dev["cells"][: s.n_fluid].long()
     i = cells // (256 * 256)
 
@@ -286,27 +285,3 @@ def test_freezing_follows_taichi(gpu):
     face = fl[-1][:, :-1]
     assert np.abs(rho[-1][:, :-1][face] - 0.997).max() < 1e-6
     assert s.force_flag == 0 and abs(s.s_v - 1.0 / (0.1 / 3 + 0.5)) < 1e-15
-
-
-@pytest.mark.parametrize("dtype", ["f32", "f16sr"])
-def test_wave_order_is_bit_exact(gpu, dtype):
-    """lbm3d_run_wave (chunks of planes, each odd update right after the even
-    update of the chunk beyond it) leaves exactly the populations and face
-    state of one launch per step, from even and odd starting steps."""
-    from taichi_lbm3d_amd.geometry import sphere_pack
-    solid = sphere_pack((96, 40, 48), porosity=0.4, rmin=3.0, rmax=7.0, seed=5, periodic=(False, True, True))
-    runs = {}
-    for wave in (0, 3, 7, 24):
-        s = pressure_solver(solid, dtype, wave=wave)
-        s.run(7)
-        s.run(9)
-        s.run(4)
-        runs[wave] = s
-        if wave:
-            assert 3 <= s._wave_plan.n_chunks <= wave
-    a = runs[0]
-    for wave in (3, 7, 24):
-        b = runs[wave]
-        assert b.steps_done == a.steps_done == 20
-        assert np.array_equal(a._dev["g"].cpu().numpy(), b._dev["g"].cpu().numpy()), wave
-        assert np.array_equal(a._dev["face_u"].cpu().numpy(), b._dev["face_u"].cpu().numpy()), wave
