@@ -334,6 +334,20 @@ int lbm3d_batch_run(void *stream, const lbm3d_params *p, int dtype, void *g, con
                     const uint32_t *face_bits, void *face_u, const uint8_t *active, int64_t t0,
                     int32_t nsteps, const lbm3d_launch *cfg);
 
+/* lbm3d_batch_run over the cells [cell_begin, cell_end) of the batch only:
+ * the samples that lie in that range advance by nsteps steps from t0, the
+ * others are untouched.  cell_begin and cell_end are multiples of
+ * LBM3D_BATCH_ALIGN (sample boundaries, or the batch's end).  Samples are
+ * independent, so a host may advance groups of samples in turn, several
+ * steps per visit, while each group's populations stay in the last-level
+ * cache; every sample's results are bit-identical to lbm3d_step on it alone.
+ * No reference counterpart (example_porous_medium.py:12-37 runs samples one
+ * after another to convergence). */
+int lbm3d_batch_run_range(void *stream, const lbm3d_params *p, int dtype, void *g, const void *desc,
+                          int32_t n_samples, const int32_t *cells, const lbm3d_row_word *rows,
+                          const uint32_t *face_bits, void *face_u, const uint8_t *active, int64_t t0,
+                          int32_t nsteps, int32_t cell_begin, int32_t cell_end, const lbm3d_launch *cfg);
+
 /* lbm3d_macros over the batch: rho[cells], v SoA [3][cells] in batch ids. */
 int lbm3d_batch_macros(void *stream, const lbm3d_params *p, int dtype, const void *g, const void *desc,
                        int32_t n_samples, const int32_t *cells, const lbm3d_row_word *rows,

@@ -129,6 +129,8 @@ SIGNATURES = {
                                 ctypes.POINTER(Launch)]),
     "lbm3d_batch_run": (_int, [_vp, _pp, _int, _vp, _vp, _i32, _vp, _vp, _vp, _vp, _vp, _i64, _i32,
                                ctypes.POINTER(Launch)]),
+    "lbm3d_batch_run_range": (_int, [_vp, _pp, _int, _vp, _vp, _i32, _vp, _vp, _vp, _vp, _vp, _i64, _i32,
+                                     _i32, _i32, ctypes.POINTER(Launch)]),
     "lbm3d_batch_macros": (_int, [_vp, _pp, _int, _vp, _vp, _i32, _vp, _vp, _vp, _vp, _i64, _vp, _vp]),
     "lbm3d_batch_reduce_workspace_bytes": (_sz, [_i64]),
     "lbm3d_batch_reduce": (_int, [_vp, _pp, _vp, _i32, _vp, _vp, _vp, _vp, _sz]),

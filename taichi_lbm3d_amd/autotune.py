@@ -88,6 +88,18 @@ def tune(batch, candidates=CANDIDATES, pairs: int = 3, use_cache: bool = True):
     "cached": bool}.  The batch's state advances by real steps while tuning.
     """
     key = _key(batch)
+    # shapes are timed step by step over the whole batch, outside the
+    # cache-blocked group schedule (restored before returning)
+    groups, live = batch.groups, batch._live
+    batch.groups, batch._live = [(0, int(batch.sizes.cells))], [0]
+    try:
+        return _tune(batch, key, candidates, pairs, use_cache)
+    finally:
+        batch.groups, batch._live = groups, live
+        batch._graph = None
+
+
+def _tune(batch, key, candidates, pairs, use_cache):
     if use_cache:
         hit = _load_cache().get(key)
         if hit:

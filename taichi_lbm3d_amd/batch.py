@@ -26,6 +26,23 @@ from . import _lib
 from .solver import CONVENTIONS, FACE_NAMES, relaxation_rates, _ptr, _ref, _torch, ctypes_void
 
 
+DEFAULT_GROUP_BYTES = 128 << 20  # population bytes per group of the cache-blocked schedule
+
+
+def group_ranges(begins, total: int, bytes_per_cell: int, group_bytes: int):
+    """[(cell_begin, cell_end)]: consecutive samples (first cells `begins`,
+    increasing, the batch ending at `total`) gathered into runs whose cells
+    take at most group_bytes; a sample larger than that forms a run of its own."""
+    bounds = list(begins) + [int(total)]
+    groups, start = [], bounds[0]
+    for s in range(1, len(bounds)):
+        if (bounds[s] - start) * bytes_per_cell > group_bytes and bounds[s - 1] > start:
+            groups.append((start, bounds[s - 1]))
+            start = bounds[s - 1]
+    groups.append((start, bounds[-1]))
+    return groups
+
+
 class BatchSolver:
     """Batch of samples with shared physics.
 
@@ -38,7 +55,7 @@ class BatchSolver:
 
     def __init__(self, solids: Sequence[np.ndarray], *, niu: float = 0.16667, force=(0.0, 0.0, 0.0),
                  faces: Optional[dict] = None, convention: str = "reference", dtype: str = "f32", device=None,
-                 use_tiles: bool = False):
+                 use_tiles: bool = False, group_bytes: Optional[int] = None, steps_per_visit: int = 32):
         if not solids:
             raise ValueError("a batch needs at least one sample")
         if convention not in CONVENTIONS:
@@ -66,8 +83,23 @@ class BatchSolver:
         # tile descriptors (LBM3D_FLAG_TILES); identical results, but slower in
         # a batch (the descriptor load waits for the sample lookup), so off
         self.use_tiles = bool(use_tiles)
+        # Cache-blocked schedule: with group_bytes > 0 and steps_per_visit > 0,
+        # consecutive samples are grouped into runs of at most group_bytes of
+        # population state, and run() advances one group by steps_per_visit
+        # steps before it moves to the next, so each group's slots stay in the
+        # 256 MB last-level cache across its steps.  Samples are independent,
+        # so the results are the same bit for bit.  group_bytes=None: groups
+        # of 128 MiB once the whole state exceeds twice that, else one group.
+        self._group_bytes_arg = group_bytes
+        self.steps_per_visit = int(steps_per_visit)
         self._graph = None
         self._build()
+        per_cell = 19 * _lib.torch_storage_dtype(self.dtype).itemsize
+        if group_bytes is None:
+            group_bytes = DEFAULT_GROUP_BYTES if int(self.sizes.cells) * per_cell > 2 * DEFAULT_GROUP_BYTES else 0
+        self.group_bytes = int(group_bytes)
+        self.groups = self._make_groups()
+        self._live = list(range(len(self.groups)))
 
     # ---- set-up ---------------------------------------------------------------
     def _params(self):
@@ -188,6 +220,8 @@ class BatchSolver:
         # every sample active: the steps get no mask (NULL), which lets even
         # steps load populations before looking their sample up
         self._all_active = True
+        if hasattr(self, "groups"):
+            self._live = list(range(len(self.groups)))
 
     def init_from_macros(self, fields):
         """Warm start: f = feq(rho, v) per sample from dense (rho (nx,ny,nz), v (nx,ny,nz,3))
@@ -225,12 +259,45 @@ class BatchSolver:
             arr[i].block, arr[i].grid_per_cu, arr[i].reverse = int(c[0]), int(c[1]), int(c[2])
         return arr
 
+    def set_schedule(self, group_bytes: int, steps_per_visit: int = 32):
+        """Switch the cache-blocked schedule: groups of at most group_bytes, or
+        one group (a launch per step over the whole batch) with group_bytes 0."""
+        self.group_bytes, self.steps_per_visit = int(group_bytes), int(steps_per_visit)
+        self.groups = self._make_groups()
+        keep = self._dev["active"].cpu().numpy().astype(bool)
+        begins = np.array([int(self.samples[s].cell_begin) for s in range(self.n_samples)])
+        self._live = [i for i, (cb, ce) in enumerate(self.groups) if keep[(begins >= cb) & (begins < ce)].any()]
+        self._graph = None
+
+    def _make_groups(self):
+        """Cell ranges of the cache-blocked schedule (group_ranges); one group when off."""
+        total = int(self.sizes.cells)
+        if self.group_bytes <= 0 or self.steps_per_visit <= 0:
+            return [(0, total)]
+        per_cell = 19 * _lib.torch_storage_dtype(self.dtype).itemsize
+        begins = [int(self.samples[s].cell_begin) for s in range(self.n_samples)]
+        return group_ranges(begins, total, per_cell, self.group_bytes)
+
     def _enqueue(self, nsteps: int, t0: int):
         d = self._dev
-        _lib.lib().call("lbm3d_batch_run", self._stream(), _ref(self.p), _lib.DTYPES[self.dtype], _ptr(d["g"]),
-                        _ptr(d["desc"]), self.n_samples, _ptr(d["cells"]), _ptr(d["rows"]), _ptr(d["face_bits"]),
-                        _ptr(d["face_u"]), None if self._all_active else _ptr(d["active"]), int(t0), int(nsteps),
-                        self._launch_arg())
+        lib = _lib.lib()
+        args = (self._stream(), _ref(self.p), _lib.DTYPES[self.dtype], _ptr(d["g"]), _ptr(d["desc"]), self.n_samples,
+                _ptr(d["cells"]), _ptr(d["rows"]), _ptr(d["face_bits"]), _ptr(d["face_u"]),
+                None if self._all_active else _ptr(d["active"]))
+        if len(self.groups) == 1:
+            lib.call("lbm3d_batch_run", *args, int(t0), int(nsteps), self._launch_arg())
+            return
+        live = [self.groups[i] for i in self._live]  # groups with an active sample
+        done, sweep = 0, 0
+        while done < nsteps:
+            m = min(self.steps_per_visit, nsteps - done)
+            # alternate the group order, so a sweep starts with the group the last one ended on
+            order = live if sweep % 2 == 0 else live[::-1]
+            for cb, ce in order:
+                lib.call("lbm3d_batch_run_range", *args, int(t0 + done), int(m), int(cb), int(ce),
+                         self._launch_arg())
+            done += m
+            sweep += 1
 
     def set_launch(self, even, odd):
         """Launch shapes (block, grid_per_cu, reverse) of even and odd steps."""
@@ -370,6 +437,12 @@ class BatchSolver:
         if self._all_active != bool(keep.all()):
             self._all_active = bool(keep.all())
             self._graph = None  # a captured graph holds the mask argument
+        # groups whose samples have all retired are no longer launched
+        begins = np.array([int(self.samples[s].cell_begin) for s in range(self.n_samples)])
+        live = [i for i, (cb, ce) in enumerate(self.groups) if keep[(begins >= cb) & (begins < ce)].any()]
+        if live != self._live:
+            self._live = live
+            self._graph = None
 
     def run_to_steady(self, tol: float = 1e-6, interval: int = 500, max_steps: int = 50_000, graph_steps: int = 0,
                       callback=None, check_every: int = 0):

@@ -1100,7 +1100,7 @@ template <typename T, bool ODD, bool FORCED, int B>
 __global__ void __launch_bounds__(B)
     k_step_batch(const KP kp, const Batch bt, T *__restrict__ g, const int32_t *__restrict__ cells,
                  const uint4 *__restrict__ rows, const uint32_t *__restrict__ face_bits,
-                 typename St<T>::R *__restrict__ face_u, int first, uint32_t n_tiles, int reverse) {
+                 typename St<T>::R *__restrict__ face_u, int first, uint32_t tile0, uint32_t n_tiles, int reverse) {
   constexpr uint32_t C = ODD ? LBM3D_XCD_CHUNK_ODD : LBM3D_XCD_CHUNK_EVEN;
   const Slots<T> sl(g, kp.S, kp.sr_seed);
   const uint32_t per_xcd = gridDim.x >> 3;
@@ -1112,7 +1112,7 @@ __global__ void __launch_bounds__(B)
       it = blockIdx.x + i * gridDim.x;
     }
     if (it >= n_tiles) break;  // every condition below is uniform in the workgroup
-    const uint32_t x0 = (reverse ? n_tiles - 1u - it : it) * (uint32_t)B;
+    const uint32_t x0 = (tile0 + (reverse ? n_tiles - 1u - it : it)) * (uint32_t)B;
     const int s = bt.chunk_sample[x0 / kChunk];
     if (bt.active && !bt.active[s]) continue;
     const SampleDesc &d = bt.desc[s];
@@ -1138,12 +1138,13 @@ template <typename T, bool ODD, bool FORCED, int B>
 __global__ void __launch_bounds__(B)
     k_step_batch1(const KP kp, const Batch bt, T *__restrict__ g, const int32_t *__restrict__ cells,
                   const uint4 *__restrict__ rows, const uint32_t *__restrict__ face_bits,
-                  typename St<T>::R *__restrict__ face_u, int first, uint32_t n_tiles, int reverse) {
+                  typename St<T>::R *__restrict__ face_u, int first, uint32_t tile0, uint32_t n_tiles,
+                  int reverse) {
   constexpr uint32_t C = ODD ? LBM3D_XCD_CHUNK_ODD : LBM3D_XCD_CHUNK_EVEN;
   uint32_t it = blockIdx.x;
   if constexpr (C > 0) it = xcd_block(C, blockIdx.x & 7u, blockIdx.x >> 3, n_tiles);
   if (it >= n_tiles) return;  // padding block of the grid
-  const uint32_t x0 = (reverse ? n_tiles - 1u - it : it) * (uint32_t)B;
+  const uint32_t x0 = (tile0 + (reverse ? n_tiles - 1u - it : it)) * (uint32_t)B;
   const uint32_t x = x0 + threadIdx.x;
   const Slots<T> sl(g, kp.S, kp.sr_seed);
   if constexpr (!ODD) {
@@ -1688,9 +1689,11 @@ bool valid_launch(const lbm3d_launch &c) {
 template <typename T, bool ODD, bool FORCED>
 void launch_batch_parity(hipStream_t s, const KP &kp, const Batch &bt, void *g, const int32_t *cells,
                          const uint4 *rows, const uint32_t *face_bits, void *face_u, int first,
-                         const lbm3d_launch &c) {
+                         const lbm3d_launch &c, uint32_t cell_begin, uint32_t cell_end) {
   using R = typename St<T>::R;
-  const uint32_t n_tiles = (uint32_t)kp.p.n_cells / (uint32_t)c.block;
+  // cell_begin and cell_end are multiples of kChunk, so of every block size
+  const uint32_t tile0 = cell_begin / (uint32_t)c.block;
+  const uint32_t n_tiles = (cell_end - cell_begin) / (uint32_t)c.block;
   constexpr uint32_t C = ODD ? LBM3D_XCD_CHUNK_ODD : LBM3D_XCD_CHUNK_EVEN;
   // XCD runs need a grid of 8 * per_xcd workgroups
   uint32_t grid = C > 0 ? (uint32_t)xcd_grid(n_tiles, C) : n_tiles;
@@ -1702,10 +1705,10 @@ void launch_batch_parity(hipStream_t s, const KP &kp, const Batch &bt, void *g, 
 #define LBM3D_BATCH_LAUNCH(B)                                                                             \
   if (c.grid_per_cu > 0)                                                                                  \
     k_step_batch<T, ODD, FORCED, B><<<grid, B, 0, s>>>(kp, bt, (T *)g, cells, rows, face_bits, (R *)face_u,  \
-                                                       first, n_tiles, rev);                               \
+                                                       first, tile0, n_tiles, rev);                        \
   else                                                                                                    \
     k_step_batch1<T, ODD, FORCED, B><<<grid, B, 0, s>>>(kp, bt, (T *)g, cells, rows, face_bits, (R *)face_u, \
-                                                        first, n_tiles, rev)
+                                                        first, tile0, n_tiles, rev)
   switch (c.block) {
     case 128: LBM3D_BATCH_LAUNCH(128); break;
     case 512: LBM3D_BATCH_LAUNCH(512); break;
@@ -1716,17 +1719,18 @@ void launch_batch_parity(hipStream_t s, const KP &kp, const Batch &bt, void *g, 
 
 template <typename T>
 int launch_batch_step(hipStream_t s, const KP &kp, const Batch &bt, void *g, const int32_t *cells, const uint4 *rows,
-                      const uint32_t *face_bits, void *face_u, int64_t t, const lbm3d_launch *cfg) {
+                      const uint32_t *face_bits, void *face_u, int64_t t, const lbm3d_launch *cfg, uint32_t cb,
+                      uint32_t ce) {
   const lbm3d_launch &c = cfg ? cfg[t & 1] : kDefaultLaunch[t & 1];
   const int first = t == 0;
   KP kpt = kp;
   kpt.sr_seed = sr_seed(kp.p, t);
   if (kp.p.force_on) {
-    if (t & 1) launch_batch_parity<T, true, true>(s, kpt, bt, g, cells, rows, face_bits, face_u, first, c);
-    else launch_batch_parity<T, false, true>(s, kpt, bt, g, cells, rows, face_bits, face_u, first, c);
+    if (t & 1) launch_batch_parity<T, true, true>(s, kpt, bt, g, cells, rows, face_bits, face_u, first, c, cb, ce);
+    else launch_batch_parity<T, false, true>(s, kpt, bt, g, cells, rows, face_bits, face_u, first, c, cb, ce);
   } else {
-    if (t & 1) launch_batch_parity<T, true, false>(s, kpt, bt, g, cells, rows, face_bits, face_u, first, c);
-    else launch_batch_parity<T, false, false>(s, kpt, bt, g, cells, rows, face_bits, face_u, first, c);
+    if (t & 1) launch_batch_parity<T, true, false>(s, kpt, bt, g, cells, rows, face_bits, face_u, first, c, cb, ce);
+    else launch_batch_parity<T, false, false>(s, kpt, bt, g, cells, rows, face_bits, face_u, first, c, cb, ce);
   }
   return check_launch("lbm3d_batch_step");
 }
@@ -2195,9 +2199,22 @@ int lbm3d_batch_init(void *stream, const lbm3d_params *p, int dtype, void *g, co
 int lbm3d_batch_run(void *stream, const lbm3d_params *p, int dtype, void *g, const void *desc, int32_t n_samples,
                     const int32_t *cells, const lbm3d_row_word *rows, const uint32_t *face_bits, void *face_u,
                     const uint8_t *active, int64_t t0, int32_t nsteps, const lbm3d_launch *cfg) {
+  if (!p) return fail(LBM3D_EINVAL, "%s", "params is NULL");
+  return lbm3d_batch_run_range(stream, p, dtype, g, desc, n_samples, cells, rows, face_bits, face_u, active, t0,
+                               nsteps, 0, p->n_cells, cfg);
+}
+
+int lbm3d_batch_run_range(void *stream, const lbm3d_params *p, int dtype, void *g, const void *desc,
+                          int32_t n_samples, const int32_t *cells, const lbm3d_row_word *rows,
+                          const uint32_t *face_bits, void *face_u, const uint8_t *active, int64_t t0, int32_t nsteps,
+                          int32_t cell_begin, int32_t cell_end, const lbm3d_launch *cfg) {
   int rc = check_batch_solver(p, dtype, n_samples, desc, g, cells, rows, face_bits);
   if (rc) return rc;
   if (t0 < 0 || nsteps < 0) return fail(LBM3D_EINVAL, "%s", "negative step index or count");
+  if (cell_begin < 0 || cell_end > p->n_cells || cell_begin >= cell_end || cell_begin % (int32_t)kChunk ||
+      cell_end % (int32_t)kChunk)
+    return fail(LBM3D_EINVAL, "%s", "cell range: 0 <= cell_begin < cell_end <= n_cells, both multiples of "
+                                    "LBM3D_BATCH_ALIGN");
   if (cfg && (!valid_launch(cfg[0]) || !valid_launch(cfg[1])))
     return fail(LBM3D_EINVAL, "%s", "launch shape: block must be 128, 256 or 512 and grid_per_cu 0..64");
   for (int f = 0; f < 6; ++f)
@@ -2206,13 +2223,16 @@ int lbm3d_batch_run(void *stream, const lbm3d_params *p, int dtype, void *g, con
   const Batch bt = batch_view(desc, n_samples, active);
   hipStream_t s = (hipStream_t)stream;
   const uint4 *r = (const uint4 *)rows;
+  const uint32_t cb = (uint32_t)cell_begin, ce = (uint32_t)cell_end;
   for (int32_t k = 0; k < nsteps; ++k) {
     const int64_t t = t0 + k;
     switch (dtype) {
-      case LBM3D_F32: rc = launch_batch_step<float>(s, kp, bt, g, cells, r, face_bits, face_u, t, cfg); break;
-      case LBM3D_F16: rc = launch_batch_step<__half>(s, kp, bt, g, cells, r, face_bits, face_u, t, cfg); break;
-      case LBM3D_F64: rc = launch_batch_step<double>(s, kp, bt, g, cells, r, face_bits, face_u, t, cfg); break;
-      default: rc = launch_batch_step<__hip_bfloat16>(s, kp, bt, g, cells, r, face_bits, face_u, t, cfg); break;
+      case LBM3D_F32: rc = launch_batch_step<float>(s, kp, bt, g, cells, r, face_bits, face_u, t, cfg, cb, ce); break;
+      case LBM3D_F16: rc = launch_batch_step<__half>(s, kp, bt, g, cells, r, face_bits, face_u, t, cfg, cb, ce); break;
+      case LBM3D_F64: rc = launch_batch_step<double>(s, kp, bt, g, cells, r, face_bits, face_u, t, cfg, cb, ce); break;
+      default:
+        rc = launch_batch_step<__hip_bfloat16>(s, kp, bt, g, cells, r, face_bits, face_u, t, cfg, cb, ce);
+        break;
     }
     if (rc) return rc;
   }

@@ -259,6 +259,28 @@ def test_batch_layout_host_only():
 
 # ---- host logic ---------------------------------------------------------------
 
+def test_batch_group_ranges():
+    """The cache-blocked schedule's groups cover the batch once, in order, on
+    sample boundaries, each within the byte budget unless one sample alone
+    exceeds it."""
+    from taichi_lbm3d_amd.batch import group_ranges
+    rng = np.random.default_rng(0)
+    for _ in range(50):
+        sizes = rng.integers(1, 40, size=int(rng.integers(1, 30))) * 512
+        begins = np.concatenate([[0], np.cumsum(sizes)[:-1]]).tolist()
+        total = int(sizes.sum())
+        budget = int(rng.integers(1, 200)) * 512 * 76
+        g = group_ranges(begins, total, 76, budget)
+        assert g[0][0] == 0 and g[-1][1] == total
+        assert all(a[1] == b[0] for a, b in zip(g[:-1], g[1:]))
+        assert all(lo < hi and lo in begins for lo, hi in g)
+        for lo, hi in g:
+            inside = [b for b in begins if lo <= b < hi]
+            assert (hi - lo) * 76 <= budget or len(inside) == 1
+    assert group_ranges([0, 512, 1024], 1536, 76, 10 ** 9) == [(0, 1536)]
+    assert group_ranges([0, 512, 1024], 1536, 76, 1) == [(0, 512), (512, 1024), (1024, 1536)]
+
+
 def test_relaxation_rates():
     from taichi_lbm3d_amd import relaxation_rates
     tau, sv, so = relaxation_rates(0.1)

@@ -221,3 +221,41 @@ def test_batch_tile_descriptors_change_nothing(gpu):
         torch.cuda.synchronize()
         out.append(b._dev["g"].cpu())
     assert torch.equal(out[0], out[1])
+
+
+@pytest.mark.parametrize("dtype,visit", [("f32", 2), ("f32", 6), ("f16sr", 4), ("f64", 3)])
+def test_batch_cache_blocked_schedule_bit_exact(gpu, dtype, visit):
+    """Samples advanced group by group, several steps per visit
+    (lbm3d_batch_run_range), end bit for bit where one launch per step over
+    the whole batch leaves them: odd and even step counts, a group budget
+    that splits the batch into several groups, graph replay for round-to-
+    nearest storage, and retired samples (a whole group no longer launched)."""
+    import torch
+    from taichi_lbm3d_amd.batch import BatchSolver
+    from taichi_lbm3d_amd.geometry import sphere_pack
+    shapes = [(24, 20, 18), (30, 16, 22), (12, 40, 20), (26, 26, 26), (18, 18, 40)]
+    solids = [sphere_pack(sh, porosity=0.4, rmin=2, rmax=5, seed=40 + i, periodic=(True, True, True))
+              for i, sh in enumerate(shapes)]
+    out = []
+    for grouped in (False, True):
+        item = {"f32": 4, "f16sr": 2, "f64": 8}[dtype]
+        kw = dict(group_bytes=8000 * 19 * item, steps_per_visit=visit) if grouped else {}
+        b = BatchSolver(solids, niu=0.1, force=(1e-5, 0.0, 0.0), dtype=dtype, **kw)
+        if grouped:
+            assert len(b.groups) >= 3, b.groups
+        b.run(7)
+        b.run(13)
+        if dtype != "f16sr":
+            b.run(24, graph_steps=8)
+        else:
+            b.run(24)
+        # retire the first two samples (a whole group when grouped): they stay as they are
+        b.retire([True, True, False, False, False])
+        if grouped:
+            assert len(b._live) < len(b.groups)
+        b.run(10)
+        torch.cuda.synchronize()
+        out.append((b._dev["g"].view(torch.uint8).cpu(), b._dev["face_u"].cpu(), b.steps_done))
+    assert out[0][2] == out[1][2] == 54
+    assert torch.equal(out[0][0], out[1][0]) and torch.equal(out[0][1], out[1][1])
+
