@@ -49,7 +49,11 @@ def test_warm_start_full_size(gpu):
     k_cold, k_warm = np.array(per["k_cold"]), np.array(per["k_warm"])
     assert np.all(np.abs(k_warm / k_cold - 1.0) < 1e-3), np.abs(k_warm / k_cold - 1.0).max()
     # the prediction starts far closer to the answer than the rest state (error 1)
-    assert r["surrogate"]["k_rel_err_max"] < 0.5
-    # and the warm start gets within 1 % of the cold permeability in fewer steps
-    assert r["warm"]["steps_to_1pct_median"] < r["cold"]["steps_to_1pct_median"]
-    assert r["speedup_to_1pct"] > 1.0
+    assert r["surrogate"]["k_rel_err_max"] < 0.2
+    # the warm start gets within 1 % of the cold permeability in fewer steps,
+    # for every sample, and reaches the steady criterion no later
+    s1c, s1w = np.array(per["steps_to_1pct_cold"]), np.array(per["steps_to_1pct_warm"])
+    assert np.all(s1w < s1c), (s1w, s1c)
+    assert r["warm"]["steps_median"] <= r["cold"]["steps_median"]
+    assert r["warm"]["steps_max"] <= r["cold"]["steps_max"]
+    assert r["speedup_to_1pct"] > 1.5

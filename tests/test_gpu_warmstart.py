@@ -1,6 +1,17 @@
 """What the trained surrogate gains: a small U-Net trained on the GPU (MFMA
 convolutions, Adam) on converged periodic packs predicts held-out packs well
-enough that a warm start reaches the cold-start permeability sooner."""
+enough that a warm start reaches the cold-start permeability sooner.
+
+In these 32^3 boxes the warm start settles 4e-4 to 1.2e-3 above the cold
+start's permeability and approaches it with a slow tail, so its 1e-6 steady
+criterion trips later (5,500-8,000 steps against 1,500-2,000).  The offset is
+the reference scheme's own: its steady state depends on the initial state (the
+FP64 CPU oracle, started from 1.2 times the cold steady field plus noise on
+one of these packs, settles 3.2e-4 above the cold start after 20,000 steps),
+and its relative size scales with 1 / F_eff (9 times smaller under the
+textbook force scaling).  At full size it is 2e-5, and steps to steady are
+asserted there, in test_gpu_config5.py, where the warm start is the faster
+one."""
 import numpy as np
 import pytest
 
@@ -24,7 +35,7 @@ def test_trained_surrogate_beats_cold_start(gpu):
     targets = [sg.make_targets(r, v, s, FX, nu) for (r, v), s in zip(fields, train_solids)]
     net = sg.UNet3D(gpu, seed=0)
     sampler = CropSampler(inputs, targets, train_solids, gpu, seed=2)
-    curve, _ = train(net, sampler, iters=400, batch=4, crop=48, log_every=100, border=8)
+    curve, _ = train(net, sampler, iters=1500, batch=4, crop=48, log_every=500, border=8)
     assert curve[-1][1] < 0.2 * curve[0][1], curve
 
     ev = [sphere_pack((32,) * 3, porosity=0.33, rmin=3.0, rmax=6.0, seed=900 + i, periodic=(True,) * 3)
@@ -40,9 +51,10 @@ def test_trained_surrogate_beats_cold_start(gpu):
     print("predicted-k error", k0_err.tolist(), "steps to 1 %: cold", s1_c.tolist(), "warm", s1_w.tolist(),
           "to steady: cold", steps_c.tolist(), "warm", steps_w.tolist())
     assert conv_w.all()
-    # both starts end at the same steady state, to the resolution of the 1e-6-per-500-steps
-    # criterion (a warm start's <vx> can oscillate, and the criterion can trip at a turning point)
-    assert np.all(np.abs(k_warm - k_cold) / k_cold < 1e-2)
-    # the cold start begins at k = 0 (error 1); the prediction starts far closer
-    assert np.all(k0_err < 0.5)
-    assert np.median(s1_w) < np.median(s1_c)
+    # both starts end at the same steady state, up to the scheme's dependence on
+    # the initial state (above)
+    assert np.all(np.abs(k_warm - k_cold) / k_cold < 2e-3)
+    # the cold start begins at k = 0 (error 1); the prediction starts within 20 %
+    assert np.all(k0_err < 0.2)
+    # and every sample gets within 1 % of the cold permeability sooner
+    assert np.all(s1_w < s1_c)
