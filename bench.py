@@ -73,7 +73,7 @@ def cpu_model():
             f"{os.cpu_count()} logical CPUs in this process's affinity mask")
 
 
-def cpu_baseline(solid, warmup=50, planned=500, budget_s=40.0):
+def cpu_baseline(solid, warmup=50, planned=500, budget_s=55.0):
     """The C restatement of the reference's four-kernel step (sparse storage,
     FP32, OpenMP over this host's cores), built -O3 -march=native on this host
     and timed on a bounded step count."""
