@@ -55,10 +55,12 @@ def test_pack256_storage_formats_to_steady_state(gpu):
         assert r[dt]["converged"] and r[dt]["nonfinite"] == 0, dt
         assert abs(r[dt]["k_rel_err"]) < 5e-3, dt
         assert r[dt]["v_rel_l2"] < 2e-2, dt
-    # stochastic rounding settles in about the FP32 step count, round to nearest does not
+    # FP16 with round to nearest needs about 3x the FP32 steps to settle (its
+    # rounding noise slows the last digits of <vx>); with stochastic rounding
+    # the count is itself random (0.8x-2x the FP32 count over runs seen)
     for dt in ("f16sr", "bf16sr"):
-        assert r[dt]["steps"] <= 1.5 * f32["steps"], dt
-    assert r["f16"]["steps"] > 2 * f32["steps"]
+        assert r[dt]["steps"] <= 2.5 * f32["steps"], dt
+    assert r["f16"]["steps"] > 2.5 * f32["steps"]
     # BF16 with round to nearest misses the target by several percent
     assert r["bf16"]["converged"] and 5e-3 < abs(r["bf16"]["k_rel_err"]) < 0.1
     # 16-bit storage moves half the bytes
