@@ -360,11 +360,11 @@ template <> struct St<float> {
 // pattern below the target format's last place, then truncate toward zero.
 // The stored value equals the exact one in expectation, so an increment far
 // below the format's spacing is kept on average instead of always rounding
-// away.  rnd: 32 random bits.
+// away.  rnd: 16 random bits (in bits 0-15).
 __device__ __forceinline__ unsigned short bf16_sr(float v, uint32_t rnd) {
   const uint32_t bits = __float_as_uint(v);
   if ((bits & 0x7F800000u) == 0x7F800000u) return (unsigned short)(bits >> 16);  // inf / nan
-  return (unsigned short)((bits + (rnd & 0xFFFFu)) >> 16);
+  return (unsigned short)((bits + rnd) >> 16);
 }
 
 __device__ __forceinline__ unsigned short half_sr(float v, uint32_t rnd) {
@@ -377,15 +377,17 @@ __device__ __forceinline__ unsigned short half_sr(float v, uint32_t rnd) {
   if (drop >= 24u) {  // |v| < 2^-24: 0 or the smallest subnormal, with probability |v| / 2^-24
     const float p = fabsf(v) * 0x1p24f;
     const unsigned short sign = (unsigned short)((bits >> 16) & 0x8000u);
-    return (float)(rnd >> 8) * 0x1p-24f < p ? (unsigned short)(sign | 1u) : sign;
+    return (float)rnd * 0x1p-16f < p ? (unsigned short)(sign | 1u) : sign;
   }
   // dither below the last place, then clear those bits: the result is a half
   // value (exact under the conversion), one grid step up with probability equal
-  // to the dropped fraction.  The bits are cleared rather than left to a
+  // to the dropped fraction (to 2^-16 of a step: the 16 random bits go to the
+  // top of a wider dropped field).  The bits are cleared rather than left to a
   // toward-zero conversion because a carry into the next binade of the
   // subnormal range keeps the same 2^-24 grid there.
   const uint32_t mask = (1u << drop) - 1u;
-  const uint32_t d = (bits + (rnd & mask)) & ~mask;
+  const uint32_t dither = drop > 16u ? rnd << (drop - 16u) : rnd;
+  const uint32_t d = (bits + (dither & mask)) & ~mask;
   return __half_as_ushort(__float2half_rn(__uint_as_float(d)));
 }
 
@@ -488,23 +490,25 @@ struct Slots {
     return St<T>::ld(r[(q + 1) >> 1], vo, (q > 0 && !(q & 1)) ? Sb : 0u);
   }
   // rs: the cell's stream (advanced by a stochastically rounded store)
-  __device__ __forceinline__ void store(int pair, uint32_t vo, uint32_t so, R v, uint32_t &rs) const {
+  // k: the store's place (0..18) among the cell's stores; each xorshift32 step
+  // feeds two stores, 16 bits each
+  __device__ __forceinline__ void store(int pair, uint32_t vo, uint32_t so, R v, uint32_t &rs, int k) const {
     if constexpr (St<T>::kStochastic) {
       if (seed) {
-        rs = xorshift32(rs);
-        St<T>::st_sr(r[pair], vo, so, v, rs);
+        if (!(k & 1)) rs = xorshift32(rs);
+        St<T>::st_sr(r[pair], vo, so, v, (k & 1) ? rs >> 16 : rs & 0xFFFFu);
         return;
       }
     }
     St<T>::st(r[pair], vo, so, v);
   }
-  __device__ __forceinline__ void st_slot(int q, uint32_t vo, R v, uint32_t &rs) const {
-    store((q + 1) >> 1, vo, (q > 0 && !(q & 1)) ? Sb : 0u, v, rs);
+  __device__ __forceinline__ void st_slot(int q, uint32_t vo, R v, uint32_t &rs, int k) const {
+    store((q + 1) >> 1, vo, (q > 0 && !(q & 1)) ? Sb : 0u, v, rs, k);
   }
   // byte offset vo relative to the pair of direction q (vo < 2 Sb)
   __device__ __forceinline__ R ld_pair(int q, uint32_t vo) const { return St<T>::ld(r[(q + 1) >> 1], vo, 0u); }
-  __device__ __forceinline__ void st_pair(int q, uint32_t vo, R v, uint32_t &rs) const {
-    store((q + 1) >> 1, vo, 0u, v, rs);
+  __device__ __forceinline__ void st_pair(int q, uint32_t vo, R v, uint32_t &rs, int k) const {
+    store((q + 1) >> 1, vo, 0u, v, rs, k);
   }
 };
 
@@ -826,11 +830,11 @@ __device__ __forceinline__ void step_cell(const KP &kp, const Geo &gm, const Slo
   if constexpr (!ODD) {
     const uint32_t xb = x * (uint32_t)sizeof(T);
 #pragma unroll
-    for (int q = 0; q < kQ; ++q) sl.st_slot(opp(q), xb, v[q], rs);
+    for (int q = 0; q < kQ; ++q) sl.st_slot(opp(q), xb, v[q], rs, q);
   } else {
-    sl.st_slot(0, off[0], v[0], rs);
+    sl.st_slot(0, off[0], v[0], rs, 0);
 #pragma unroll
-    for (int q = 1; q < kQ; ++q) sl.st_pair(q, off[q], v[q], rs);
+    for (int q = 1; q < kQ; ++q) sl.st_pair(q, off[q], v[q], rs, q);
   }
 }
 
@@ -880,7 +884,7 @@ __device__ __forceinline__ void init_cell(const KP &kp, const Geo &gm, const Slo
   const uint32_t xb = x * (uint32_t)sizeof(T);
   uint32_t rs = 0u;  // init rounds to nearest (sl has no seed)
 #pragma unroll
-  for (int q = 0; q < kQ; ++q) sl.st_slot(q, xb, feq_shift<R>(q, rho, ux, uy, uz, uu), rs);
+  for (int q = 0; q < kQ; ++q) sl.st_slot(q, xb, feq_shift<R>(q, rho, ux, uy, uz, uu), rs, q);
   int i, j, k;
   coords(gm, (uint32_t)cells[x], i, j, k);
   const int owner = face_owner(p, gm, i, j, k);
@@ -1165,7 +1169,7 @@ __global__ void __launch_bounds__(B)
       const uint32_t xb = x * (uint32_t)sizeof(T);
       uint32_t rs = sl.stream(x - d.cell_begin);
 #pragma unroll
-      for (int q = 0; q < kQ; ++q) sl.st_slot(opp(q), xb, v[q], rs);
+      for (int q = 0; q < kQ; ++q) sl.st_slot(opp(q), xb, v[q], rs, q);
       return;
     }
   }
