@@ -39,7 +39,7 @@
 extern "C" {
 #endif
 
-#define LBM3D_ABI_VERSION 12
+#define LBM3D_ABI_VERSION 13
 
 /* storage types for populations; F32, F16 and BF16 compute in FP32, F64 in FP64 */
 #define LBM3D_F32 0
@@ -98,6 +98,8 @@ typedef struct lbm3d_params {
   double guo_a, guo_b;     /* force-term scalings: SP uses 1/3 and 1/9 (SP:236) */
   double rho_bc[6];        /* rho_bcxl ... (SP:23-28) */
   double u_bc[6][3];       /* vx_bcxl, vy_bcxl, vz_bcxl ... (SP:23-28) */
+  int32_t band_rows;       /* order of the fluid list (lbm3d_compact_fill): bands of this many
+                              rows j; 0 or >= ny: C order */
 } lbm3d_params;
 
 /* One 32-cell word of a grid row (i, j): bit b of `mask` set when cell
@@ -147,7 +149,15 @@ int lbm3d_compact_count(void *stream, const lbm3d_params *p, const int8_t *solid
 
 /* Pass 2 (after pass 1 with the same workspace; p->n_cells = the count):
  * cells[n_cells] = C-order dense index of each fluid cell, and, when d2c is
- * not NULL, d2c[nx*ny*nz] = compact id or -1. */
+ * not NULL, d2c[nx*ny*nz] = compact id or -1.  Compact ids follow the list
+ * order of p->band_rows = R: the grid rows (i, j) are cut into bands of R
+ * consecutive j (the last band may be narrower), and the fluid cells are
+ * numbered band after band, inside a band plane after plane (i), then row
+ * after row (j), k fastest.  R = 0 or R >= ny is C order (k fastest, then j,
+ * then i).  Every row's fluid cells get consecutive ids in k order, which is
+ * what the row table encodes, so every other function takes either order;
+ * the odd step's neighbours in planes i +- 1 are R rows of cells away in the
+ * list instead of ny rows. */
 int lbm3d_compact_fill(void *stream, const lbm3d_params *p, const int8_t *solid, int32_t *d2c,
                        int32_t *cells, void *workspace, size_t workspace_bytes);
 

@@ -7,7 +7,11 @@ pushing ``F[ip][s] = f[i][s]`` when ``ip`` is fluid and bouncing back into
 information as a numbered fluid list and a table; these functions restate it
 two ways (plain loops, and numpy) so the GPU build can be checked bit for bit.
 
-Numbering: fluid cells in C order.  Face records: a fluid node on a face with
+Numbering: fluid cells in list order.  With band_rows = R > 0 and R < ny the
+grid rows (i, j) are cut into bands of R consecutive j (the last band may be
+narrower) and the cells are numbered band after band, inside a band plane
+after plane (i), row after row (j), k fastest; R = 0 or R >= ny is C order.
+Face records: a fluid node on a face with
 a pressure or velocity condition belongs to the last such face of x-left,
 x-right, y-left, y-right, z-left, z-right, the order Boundary_condition
 writes them (SP:274-370); its record is 1, plus 2 when that face's inward
@@ -47,10 +51,20 @@ def face_owner(solid: np.ndarray, bc_mode) -> tuple[np.ndarray, np.ndarray]:
     return owner, tag
 
 
-def compact(solid: np.ndarray):
-    """(cells, dense_to_compact) with numpy: fluid cells (value 0) in C order."""
-    fluid = (np.asarray(solid) == 0).reshape(-1)
-    cells = np.flatnonzero(fluid).astype(np.int32)
+def list_order(shape, band_rows: int = 0) -> np.ndarray:
+    """C-order dense indices of every cell of the grid in list order (numpy)."""
+    nx, ny, nz = shape
+    R = ny if band_rows <= 0 or band_rows >= ny else int(band_rows)
+    idx = np.arange(nx * ny * nz, dtype=np.int64).reshape(nx, ny, nz)
+    return np.concatenate([idx[:, b:b + R, :].reshape(-1) for b in range(0, ny, R)])
+
+
+def compact(solid: np.ndarray, band_rows: int = 0):
+    """(cells, dense_to_compact) with numpy: fluid cells (value 0) in list order."""
+    solid = np.asarray(solid)
+    fluid = (solid == 0).reshape(-1)
+    order = list_order(solid.shape, band_rows)
+    cells = order[fluid[order]].astype(np.int32)
     d2c = np.full(fluid.size, -1, np.int32)
     d2c[cells] = np.arange(cells.size, dtype=np.int32)
     return cells, d2c
@@ -88,18 +102,20 @@ def neighbors(solid: np.ndarray, cells: np.ndarray, d2c: np.ndarray) -> np.ndarr
     return out
 
 
-def compact_loops(solid, bc_mode=(0,) * 6):
+def compact_loops(solid, bc_mode=(0,) * 6, band_rows: int = 0):
     """The same numbering, face records and table with plain Python loops
     (small grids only), following periodic_index (SP:247-257), streaming1
     (SP:259-269) and the face order of Boundary_condition (SP:274-370) literally."""
     solid = np.asarray(solid)
     nx, ny, nz = solid.shape
+    R = ny if band_rows <= 0 or band_rows >= ny else int(band_rows)
     order = []
-    for i in range(nx):
-        for j in range(ny):
-            for k in range(nz):
-                if solid[i, j, k] == 0:
-                    order.append((i, j, k))
+    for j0 in range(0, ny, R):
+        for i in range(nx):
+            for j in range(j0, min(j0 + R, ny)):
+                for k in range(nz):
+                    if solid[i, j, k] == 0:
+                        order.append((i, j, k))
     ids = {c: x for x, c in enumerate(order)}
     rec = []
     for (i, j, k) in order:

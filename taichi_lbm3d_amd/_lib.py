@@ -15,7 +15,7 @@ LIB_PATH = os.path.join(_HERE, "liblbm3d.so")
 SOURCES = [os.path.join(_HERE, "csrc", name) for name in ("lbm3d.hip", "conv3d.hip", "geo_io.hip")]
 INCLUDE = os.path.join(_ROOT, "include")
 
-ABI_VERSION = 12
+ABI_VERSION = 13
 F32, F16, BF16, F64 = 0, 1, 2, 3
 FLAG_STOCHASTIC_ROUNDING = 1
 FLAG_TILES = 2
@@ -65,6 +65,7 @@ class Params(ctypes.Structure):
         ("guo_a", ctypes.c_double), ("guo_b", ctypes.c_double),
         ("rho_bc", ctypes.c_double * 6),
         ("u_bc", (ctypes.c_double * 3) * 6),
+        ("band_rows", ctypes.c_int32),
     ]
 
 

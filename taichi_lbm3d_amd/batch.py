@@ -50,12 +50,13 @@ class BatchSolver:
     niu, force  -- as the reference's niu and (fx, fy, fz) (SP:13-17)
     faces       -- optional {face name: ("pressure", rho) | ("velocity", (vx, vy, vz))},
                    face names as FACE_NAMES; unnamed faces are periodic
-    convention, dtype, device -- as LB3D_Solver_Single_Phase
+    convention, dtype, device, band_rows -- as LB3D_Solver_Single_Phase
     """
 
     def __init__(self, solids: Sequence[np.ndarray], *, niu: float = 0.16667, force=(0.0, 0.0, 0.0),
                  faces: Optional[dict] = None, convention: str = "reference", dtype: str = "f32", device=None,
-                 use_tiles: bool = False, group_bytes: Optional[int] = None, steps_per_visit: int = 32):
+                 use_tiles: bool = False, group_bytes: Optional[int] = None, steps_per_visit: int = 32,
+                 band_rows: int = 32):
         if not solids:
             raise ValueError("a batch needs at least one sample")
         if convention not in CONVENTIONS:
@@ -83,6 +84,9 @@ class BatchSolver:
         # tile descriptors (LBM3D_FLAG_TILES); identical results, but slower in
         # a batch (the descriptor load waits for the sample lookup), so off
         self.use_tiles = bool(use_tiles)
+        # order of every sample's fluid list, as LB3D_Solver_Single_Phase.band_rows
+        # (the same default, so a sample's slots match its single-grid run's)
+        self.band_rows = int(band_rows)
         # Cache-blocked schedule: with group_bytes > 0 and steps_per_visit > 0,
         # consecutive samples are grouped into runs of at most group_bytes of
         # population state, and run() advances one group by steps_per_visit
@@ -110,6 +114,7 @@ class BatchSolver:
         p.fx, p.fy, p.fz = self.fx, self.fy, self.fz
         p.force_on = 1 if (self.fx != 0 or self.fy != 0 or self.fz != 0) else 0
         p.flags = _lib.STORAGE_FLAGS.get(self.dtype, 0)
+        p.band_rows = self.band_rows
         for f, name in enumerate(FACE_NAMES):
             p.face_u_off[f] = -1
             p.rho_bc[f] = 1.0

@@ -40,14 +40,24 @@ constexpr int kWinWords = 384;       // LDS row window: 3 planes x (J+3) rows x 
 
 // Build-time knobs (defaults are the measured best on MI355X).
 #ifndef LBM3D_XCD_CHUNK_EVEN
-#define LBM3D_XCD_CHUNK_EVEN 0  // C > 0: each XCD takes runs of C consecutive blocks on even steps
+#define LBM3D_XCD_CHUNK_EVEN 128  // C > 0: each XCD takes runs of C consecutive blocks on even steps
 #endif
 #ifndef LBM3D_XCD_CHUNK_ODD
 // ... and on odd steps (0: the hardware's round robin).  An odd step's cells
-// read and write slot lines shared with the cells one plane away (and the
-// bounce-back slots of their own lines); 256-block runs keep most of those
-// sharers on one XCD's L2: 239 instead of 251 us per odd step on a 256^3 pack.
-#define LBM3D_XCD_CHUNK_ODD 256
+// read and write slot lines shared with the cells one plane away (R rows of
+// cells away in a list in bands of R rows) and the bounce-back slots of their
+// own lines; runs of consecutive blocks keep most of those sharers on one
+// XCD's L2.  On a 256^3 pack in bands of 32 rows, runs of 128 blocks on both
+// parities: 166 us per step against 169-170 with runs of 256 on odd steps only.
+#define LBM3D_XCD_CHUNK_ODD 128
+#endif
+// The batch kernels' runs (samples in C order): 256 blocks on odd steps, the
+// hardware's round robin on even steps.
+#ifndef LBM3D_BATCH_XCD_CHUNK_EVEN
+#define LBM3D_BATCH_XCD_CHUNK_EVEN 0
+#endif
+#ifndef LBM3D_BATCH_XCD_CHUNK_ODD
+#define LBM3D_BATCH_XCD_CHUNK_ODD 256
 #endif
 #ifndef LBM3D_STEP_MINB
 #define LBM3D_STEP_MINB 1
@@ -131,6 +141,7 @@ struct Geo {
   uint32_t plane[6];       // cells per face plane
   int32_t face_u_off[6];   // first face_u element of each pressure face's plane
   uint32_t n_row_words;    // nx * ny * W
+  uint32_t band;           // rows j per band of the fluid list's order, in [1, ny] (ny: C order)
 };
 
 struct KP {
@@ -151,7 +162,7 @@ void magic(uint32_t d, uint32_t &m, uint32_t &s) {
   s = 31 + l;
 }
 
-Geo make_geo(int32_t nx, int32_t ny, int32_t nz, const int32_t face_u_off[6]) {
+Geo make_geo(int32_t nx, int32_t ny, int32_t nz, const int32_t face_u_off[6], int32_t band_rows) {
   Geo g;
   g.nx = nx;
   g.ny = ny;
@@ -165,6 +176,7 @@ Geo make_geo(int32_t nx, int32_t ny, int32_t nz, const int32_t face_u_off[6]) {
   g.plane[4] = g.plane[5] = (uint32_t)nx * ny;
   for (int f = 0; f < 6; ++f) g.face_u_off[f] = face_u_off[f];
   g.n_row_words = (uint32_t)nx * (uint32_t)ny * g.W;
+  g.band = (uint32_t)(band_rows <= 0 || band_rows >= ny ? ny : band_rows);
   return g;
 }
 
@@ -189,7 +201,7 @@ KP make_kp(const lbm3d_params &p) {
   fill_phys(k.pf, p);
   fill_phys(k.pd, p);
   k.S = slot_stride((uint32_t)(p.n_cells > 0 ? p.n_cells : 0));
-  k.geo = make_geo(p.nx, p.ny, p.nz, p.face_u_off);
+  k.geo = make_geo(p.nx, p.ny, p.nz, p.face_u_off, p.band_rows);
   k.sr_seed = 0u;
   return k;
 }
@@ -912,7 +924,9 @@ __device__ __forceinline__ RowView row_window(const Geo &gm, const int32_t *__re
   coords(gm, (uint32_t)cells[x0], i0, j0, k0);
   coords(gm, (uint32_t)cells[xl], i1, j1, k1);
   const uint32_t JW = (uint32_t)(j1 - j0) + 3u;
-  if (i0 == i1 && JW * 3u * gm.W <= (uint32_t)kWinWords) {
+  // one plane's rows of one band (a block of a small grid can span a whole
+  // cycle of a band's planes and end in the plane it started in)
+  if (i0 == i1 && (uint32_t)j0 / gm.band == (uint32_t)j1 / gm.band && JW * 3u * gm.W <= (uint32_t)kWinWords) {
     const uint32_t per_plane = JW * gm.W, cnt = 3u * per_plane;
     for (uint32_t e = threadIdx.x; e < cnt; e += B) {
       const uint32_t di = e / per_plane, r = e - di * per_plane;
@@ -1105,7 +1119,7 @@ __global__ void __launch_bounds__(B)
     k_step_batch(const KP kp, const Batch bt, T *__restrict__ g, const int32_t *__restrict__ cells,
                  const uint4 *__restrict__ rows, const uint32_t *__restrict__ face_bits,
                  typename St<T>::R *__restrict__ face_u, int first, uint32_t tile0, uint32_t n_tiles, int reverse) {
-  constexpr uint32_t C = ODD ? LBM3D_XCD_CHUNK_ODD : LBM3D_XCD_CHUNK_EVEN;
+  constexpr uint32_t C = ODD ? LBM3D_BATCH_XCD_CHUNK_ODD : LBM3D_BATCH_XCD_CHUNK_EVEN;
   const Slots<T> sl(g, kp.S, kp.sr_seed);
   const uint32_t per_xcd = gridDim.x >> 3;
   for (uint32_t i = 0;; ++i) {
@@ -1144,7 +1158,7 @@ __global__ void __launch_bounds__(B)
                   const uint4 *__restrict__ rows, const uint32_t *__restrict__ face_bits,
                   typename St<T>::R *__restrict__ face_u, int first, uint32_t tile0, uint32_t n_tiles,
                   int reverse) {
-  constexpr uint32_t C = ODD ? LBM3D_XCD_CHUNK_ODD : LBM3D_XCD_CHUNK_EVEN;
+  constexpr uint32_t C = ODD ? LBM3D_BATCH_XCD_CHUNK_ODD : LBM3D_BATCH_XCD_CHUNK_EVEN;
   uint32_t it = blockIdx.x;
   if constexpr (C > 0) it = xcd_block(C, blockIdx.x & 7u, blockIdx.x >> 3, n_tiles);
   if (it >= n_tiles) return;  // padding block of the grid
@@ -1233,15 +1247,43 @@ __global__ void __launch_bounds__(kBlock)
 }
 
 // ---------------------------------------------------------------------------
-// Compaction: tiles of kTile cells (kItems per thread); per tile the count of
-// fluid cells, an exclusive scan over tiles, then a block-level scan inside
-// each tile to number the cells in C order.
+// Compaction: tiles of kTile positions (kItems per thread) of the list order;
+// per tile the count of fluid cells, an exclusive scan over tiles, then a
+// block-level scan inside each tile to number the cells in list order.
+//
+// List order: the grid rows (i, j) in bands of R consecutive j (the last band
+// may be narrower); band after band, and inside a band plane after plane
+// (i), row after row (j), k fastest.  R >= ny is C order.  Every row's cells
+// stay consecutive and in k order, which is all the row words assume; a
+// cell's neighbours in the next and previous planes are then R rows of cells
+// away in the list instead of a whole plane.
 // ---------------------------------------------------------------------------
 constexpr int kItems = 4;
 constexpr int kTile = kBlock * kItems;
 
-__global__ void __launch_bounds__(kBlock) k_compact_count(const int8_t *__restrict__ solid, int64_t n_dense,
-                                                          int32_t *__restrict__ tile_cnt) {
+struct ListOrder {
+  uint32_t nx, ny, nz, R;  // R: band rows, clamped to [1, ny]
+};
+
+ListOrder list_order(const lbm3d_params &p) {
+  const int32_t R = p.band_rows <= 0 || p.band_rows >= p.ny ? p.ny : p.band_rows;
+  return ListOrder{(uint32_t)p.nx, (uint32_t)p.ny, (uint32_t)p.nz, (uint32_t)R};
+}
+
+// dense C-order index of list position pos (pos < nx ny nz < 2^31)
+__device__ __forceinline__ uint32_t list_dense(const ListOrder &o, uint32_t pos) {
+  if (o.R == o.ny) return pos;
+  const uint32_t row = pos / o.nz, k = pos - row * o.nz;
+  const uint32_t band_rows = o.nx * o.R;  // rows of a full band
+  const uint32_t jb = row / band_rows;
+  const uint32_t Rb = min(o.R, o.ny - jb * o.R);
+  const uint32_t r = row - jb * band_rows;
+  const uint32_t i = r / Rb, j = jb * o.R + (r - i * Rb);
+  return (i * o.ny + j) * o.nz + k;
+}
+
+__global__ void __launch_bounds__(kBlock) k_compact_count(const int8_t *__restrict__ solid, const ListOrder o,
+                                                          int64_t n_dense, int32_t *__restrict__ tile_cnt) {
   __shared__ int s_cnt;
   if (threadIdx.x == 0) s_cnt = 0;
   __syncthreads();
@@ -1250,7 +1292,7 @@ __global__ void __launch_bounds__(kBlock) k_compact_count(const int8_t *__restri
   for (int it = 0; it < kItems; ++it) {
     const int64_t c = base + (int64_t)threadIdx.x * kItems + it;
     if (c >= n_dense) break;
-    cnt += solid[c] == 0;  // fluid iff solid == 0 (SP:225)
+    cnt += solid[list_dense(o, (uint32_t)c)] == 0;  // fluid iff solid == 0 (SP:225)
   }
   if (cnt) atomicAdd(&s_cnt, cnt);
   __syncthreads();
@@ -1292,17 +1334,19 @@ __global__ void __launch_bounds__(kBlock) k_compact_scan(int32_t *__restrict__ t
   if (threadIdx.x == 0) count[0] = carry;
 }
 
-__global__ void __launch_bounds__(kBlock) k_compact_fill(const int8_t *__restrict__ solid, int64_t n_dense,
-                                                         const int32_t *__restrict__ tile_off,
+__global__ void __launch_bounds__(kBlock) k_compact_fill(const int8_t *__restrict__ solid, const ListOrder lo,
+                                                         int64_t n_dense, const int32_t *__restrict__ tile_off,
                                                          int32_t *__restrict__ d2c, int32_t *__restrict__ cells) {
   __shared__ int s_warp[kBlock / 64];
   const int64_t base = (int64_t)blockIdx.x * kTile;
   bool fl[kItems];
+  uint32_t dn[kItems];
   int cnt = 0;
 #pragma unroll
   for (int it = 0; it < kItems; ++it) {
     const int64_t c = base + (int64_t)threadIdx.x * kItems + it;
-    fl[it] = c < n_dense && solid[c] == 0;
+    dn[it] = c < n_dense ? list_dense(lo, (uint32_t)c) : 0u;
+    fl[it] = c < n_dense && solid[dn[it]] == 0;
     cnt += fl[it];
   }
   int o = block_inclusive_scan(cnt, s_warp) - cnt + tile_off[blockIdx.x];
@@ -1311,8 +1355,8 @@ __global__ void __launch_bounds__(kBlock) k_compact_fill(const int8_t *__restric
     const int64_t c = base + (int64_t)threadIdx.x * kItems + it;
     if (c >= n_dense) break;
     const int32_t id = fl[it] ? o++ : -1;
-    if (d2c) d2c[c] = id;
-    if (id >= 0) cells[id] = (int32_t)c;
+    if (d2c) d2c[dn[it]] = id;
+    if (id >= 0) cells[id] = (int32_t)dn[it];
   }
 }
 
@@ -1360,7 +1404,10 @@ __global__ void __launch_bounds__(kBlock) k_build_tiles(const Geo gm, const int3
   coords(gm, (uint32_t)cells[x0], i0, j0, k0);
   coords(gm, (uint32_t)cells[xl], i1, j1, k1);
   const uint32_t J = (uint32_t)(j1 - j0);
-  const bool ok = i0 == i1 && 3u * (J + 3u) * gm.W <= (uint32_t)kWinWords && (J + 1u) * gm.W <= 64u;
+  // the block lies in rows j0 .. j1 of one plane when its first and last cells
+  // are in the same plane and the same band of the list order
+  const bool ok = i0 == i1 && (uint32_t)j0 / gm.band == (uint32_t)j1 / gm.band &&
+                  3u * (J + 3u) * gm.W <= (uint32_t)kWinWords && (J + 1u) * gm.W <= 64u;
   tiles[t] = make_uint4((uint32_t)i0, (uint32_t)j0, ok ? J : kNoTile, 0u);
 }
 
@@ -1540,6 +1587,7 @@ int check_params(const lbm3d_params *p) {
   if (p->nx < 1 || p->ny < 1 || p->nz < 1) return fail(LBM3D_EINVAL, "%s", "grid extents must be >= 1");
   if ((int64_t)p->nx * p->ny * p->nz > INT32_MAX)
     return fail(LBM3D_ESIZE, "%s", "grid has more than 2^31-1 cells");
+  if (p->band_rows < 0) return fail(LBM3D_EINVAL, "%s", "band_rows must be >= 0");
   for (int f = 0; f < 6; ++f) {
     if (p->bc_mode[f] == 1 || p->bc_mode[f] == 2) {
       const int n = f < 2 ? p->nx : (f < 4 ? p->ny : p->nz);
@@ -1698,7 +1746,7 @@ void launch_batch_parity(hipStream_t s, const KP &kp, const Batch &bt, void *g, 
   // cell_begin and cell_end are multiples of kChunk, so of every block size
   const uint32_t tile0 = cell_begin / (uint32_t)c.block;
   const uint32_t n_tiles = (cell_end - cell_begin) / (uint32_t)c.block;
-  constexpr uint32_t C = ODD ? LBM3D_XCD_CHUNK_ODD : LBM3D_XCD_CHUNK_EVEN;
+  constexpr uint32_t C = ODD ? LBM3D_BATCH_XCD_CHUNK_ODD : LBM3D_BATCH_XCD_CHUNK_EVEN;
   // XCD runs need a grid of 8 * per_xcd workgroups
   uint32_t grid = C > 0 ? (uint32_t)xcd_grid(n_tiles, C) : n_tiles;
   if (c.grid_per_cu > 0) {
@@ -1884,7 +1932,7 @@ int lbm3d_compact_count(void *stream, const lbm3d_params *p, const int8_t *solid
     return fail(LBM3D_EINVAL, "%s", "compaction workspace too small");
   const int64_t tiles = (n + kTile - 1) / kTile;
   hipStream_t s = (hipStream_t)stream;
-  k_compact_count<<<(unsigned)tiles, kBlock, 0, s>>>(solid, n, (int32_t *)workspace);
+  k_compact_count<<<(unsigned)tiles, kBlock, 0, s>>>(solid, list_order(*p), n, (int32_t *)workspace);
   if ((rc = check_launch("lbm3d_compact_count"))) return rc;
   k_compact_scan<<<1, kBlock, 0, s>>>((int32_t *)workspace, tiles, count);
   return check_launch("lbm3d_compact_count/scan");
@@ -1901,8 +1949,8 @@ int lbm3d_compact_fill(void *stream, const lbm3d_params *p, const int8_t *solid,
   if (workspace_bytes < lbm3d_compact_workspace_bytes(n))
     return fail(LBM3D_EINVAL, "%s", "compaction workspace too small");
   const int64_t tiles = (n + kTile - 1) / kTile;
-  k_compact_fill<<<(unsigned)tiles, kBlock, 0, (hipStream_t)stream>>>(solid, n, (const int32_t *)workspace, d2c,
-                                                                      cells);
+  k_compact_fill<<<(unsigned)tiles, kBlock, 0, (hipStream_t)stream>>>(solid, list_order(*p), n,
+                                                                      (const int32_t *)workspace, d2c, cells);
   return check_launch("lbm3d_compact_fill");
 }
 
@@ -1914,7 +1962,7 @@ int lbm3d_build_rows_at(void *stream, const lbm3d_params *p, const int8_t *solid
   if (id_offset < 0) return fail(LBM3D_EINVAL, "%s", "negative id offset");
   if ((int64_t)id_offset + (int64_t)p->nx * p->ny * p->nz > INT32_MAX)
     return fail(LBM3D_ESIZE, "%s", "id offset plus cells exceeds int32");
-  const Geo gm = make_geo(p->nx, p->ny, p->nz, p->face_u_off);
+  const Geo gm = make_geo(p->nx, p->ny, p->nz, p->face_u_off, p->band_rows);
   const int64_t n = lbm3d_row_words(p);
   k_build_rows<<<(unsigned)((n + kBlock - 1) / kBlock), kBlock, 0, (hipStream_t)stream>>>(gm, solid, d2c, id_offset,
                                                                                          (uint4 *)rows);
@@ -1937,7 +1985,7 @@ int lbm3d_build_tiles(void *stream, const lbm3d_params *p, const int32_t *cells,
   if ((rc = check_cells(p))) return rc;
   if (p->n_cells == 0) return LBM3D_OK;
   if (!cells || !tiles) return fail(LBM3D_EINVAL, "%s", "NULL argument to lbm3d_build_tiles");
-  const Geo gm = make_geo(p->nx, p->ny, p->nz, p->face_u_off);
+  const Geo gm = make_geo(p->nx, p->ny, p->nz, p->face_u_off, p->band_rows);
   const int64_t nt = lbm3d_tile_words(p);
   k_build_tiles<<<(unsigned)((nt + kBlock - 1) / kBlock), kBlock, 0, (hipStream_t)stream>>>(
       gm, cells, (uint32_t)p->n_cells, (uint4 *)tiles);
@@ -2163,7 +2211,7 @@ int lbm3d_batch_desc(const lbm3d_params *p, const lbm3d_sample *samples, int32_t
       off[f] = p->bc_mode[f] == 1 ? (int32_t)fu : -1;
       if (p->bc_mode[f] == 1) fu += 3 * lbm3d_face_plane(&q, f);
     }
-    d[s].geo = make_geo(sm.nx, sm.ny, sm.nz, off);
+    d[s].geo = make_geo(sm.nx, sm.ny, sm.nz, off, p->band_rows);
     d[s].cell_begin = (uint32_t)sm.cell_begin;
     d[s].cell_end = (uint32_t)(sm.cell_begin + (sm.n_cells > 0 ? sm.n_cells : 0));
     const int64_t sp = sm.n_cells > 0 ? sm.n_cells : 1;

@@ -1,7 +1,7 @@
 """Short run of the step kernels for hardware-counter profiles.
 
     python taichi_lbm3d_amd/profile_step.py [--n 256] [--steps 20] [--dtype f32]
-                                            [--calib-copy-mib 2048]
+                                            [--calib-copy-mib 2048] [--band-rows R]
 
 Runs the 256^3 sphere-pack workload of bench.py (seed 0, pressure x faces,
 niu 0.1) for a few steps and prints the per-kernel time of each parity.
@@ -30,6 +30,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=4)
     ap.add_argument("--dtype", default="f32", choices=["f32", "f16", "bf16", "f16sr", "bf16sr"])
     ap.add_argument("--calib-copy-mib", type=int, default=0)
+    ap.add_argument("--band-rows", type=int, default=None, help="fluid-list bands (default: the solver's)")
     args = ap.parse_args()
 
     import torch
@@ -50,6 +51,8 @@ def main():
     solid = sphere_pack((args.n,) * 3, porosity=0.35, rmin=6.0, rmax=14.0, seed=args.seed,
                         periodic=(False, True, True))
     s = LB3D_Solver_Single_Phase(*solid.shape, sparse_storage=True, dtype=args.dtype)
+    if args.band_rows is not None:
+        s.band_rows = args.band_rows
     s.solid.from_numpy(solid)
     s.niu = 0.1
     s.set_bc_rho_x0(1.0005)
@@ -65,7 +68,7 @@ def main():
     torch.cuda.synchronize()
     dts = np.array([ev[i].elapsed_time(ev[i + 1]) for i in range(args.steps)]) * 1e3
     par = np.array([(args.warmup + i) % 2 for i in range(args.steps)])
-    print(json.dumps({"fluid_cells": s.n_fluid, "dtype": args.dtype, "steps": args.steps,
+    print(json.dumps({"fluid_cells": s.n_fluid, "dtype": args.dtype, "band_rows": s.band_rows, "steps": args.steps,
                       "even_us": float(np.median(dts[par == 0])), "odd_us": float(np.median(dts[par == 1]))}))
 
 

@@ -66,6 +66,47 @@ def test_compaction_loops_equal_numpy(seed):
     np.testing.assert_array_equal(unpacked, rec)
 
 
+@pytest.mark.parametrize("seed", range(4))
+@pytest.mark.parametrize("band_rows", [1, 2, 3, 5])
+def test_band_order_loops_equal_numpy(seed, band_rows):
+    """The list in bands of R rows: numpy and plain loops agree on the numbering,
+    neighbour table and face records, also when R does not divide ny."""
+    rng = np.random.default_rng(100 + seed)
+    shape = (int(rng.integers(2, 6)), int(rng.integers(4, 12)), int(rng.integers(2, 6)))
+    solid = rng.choice(np.array([0, 1, -1], np.int8), size=shape, p=[0.7, 0.2, 0.1])
+    modes = tuple(int(x) for x in rng.integers(0, 3, 6))
+    cells, d2c = oc.compact(solid, band_rows)
+    c2, rec2, nb2 = oc.compact_loops(solid, modes, band_rows)
+    np.testing.assert_array_equal(cells, c2)
+    np.testing.assert_array_equal(oc.neighbors(solid, cells, d2c), nb2)
+    np.testing.assert_array_equal(oc.face_records(solid, cells, modes), rec2)
+
+
+@pytest.mark.parametrize("shape,band_rows", [((3, 10, 4), 4), ((5, 7, 3), 2), ((2, 9, 6), 3), ((4, 6, 5), 6)])
+def test_band_order_closed_form(shape, band_rows):
+    """All fluid: the id of (i, j, k) is jb nx R nz + i Rb nz + (j - jb R) nz + k,
+    jb = j // R and Rb = min(R, ny - jb R) the rows of its band; R >= ny and
+    R = 0 give C order.  With solids, every row's fluid cells keep consecutive
+    ids in k order (what the row table encodes)."""
+    nx, ny, nz = shape
+    R = band_rows
+    _, d2c = oc.compact(np.zeros(shape, np.int8), R)
+    i, j, k = np.meshgrid(np.arange(nx), np.arange(ny), np.arange(nz), indexing="ij")
+    jb = j // R
+    Rb = np.minimum(R, ny - jb * R)
+    expect = jb * nx * R * nz + i * Rb * nz + (j - jb * R) * nz + k
+    np.testing.assert_array_equal(d2c.reshape(shape), expect)
+    solid = (np.random.default_rng(3).random(shape) < 0.3).astype(np.int8)
+    for r in (0, ny, ny + 3):
+        np.testing.assert_array_equal(oc.compact(solid, r)[0], np.flatnonzero(solid.reshape(-1) == 0))
+    _, d2c = oc.compact(solid, R)
+    ids = d2c.reshape(shape)
+    for a in range(nx):
+        for b in range(ny):
+            row = ids[a, b][ids[a, b] >= 0]
+            assert np.all(np.diff(row) == 1)
+
+
 def test_compaction_cavity_checksum():
     solid, _ = load_mask("geo_cavity.npz", (50, 50, 50))
     g = np.load(os.path.join(GOLDEN, "cavity_compaction.npz"))

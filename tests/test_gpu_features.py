@@ -159,7 +159,8 @@ def test_export_vtk_roundtrip(gpu, tmp_path, monkeypatch):
 def test_full_size_pack_invariants(gpu):
     """256^3 sphere pack, all faces periodic, no force: total mass conserved to
     FP32 round-off over 400 steps (both AA parities), fields finite, and the
-    neighbour ids bit-exact against the brute-force table."""
+    fluid list (in the solver's default bands of rows) and neighbour ids
+    bit-exact against the brute-force table."""
     from taichi_lbm3d_amd import LB3D_Solver_Single_Phase
     from taichi_lbm3d_amd.geometry import sphere_pack
     from oracle import compaction as oc
@@ -168,7 +169,7 @@ def test_full_size_pack_invariants(gpu):
     s.solid.from_numpy(solid)
     s.niu = 0.1
     s.init_simulation()
-    cells, d2c = oc.compact(solid)
+    cells, d2c = oc.compact(solid, s.band_rows)
     np.testing.assert_array_equal(s._dev["cells"].cpu().numpy(), cells)
     np.testing.assert_array_equal(s.neighbor_table(), oc.neighbors(solid, cells, d2c))
     rng = np.random.default_rng(1)

@@ -26,10 +26,13 @@ def rel_l2(a, b, floor=1e-9):
     return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), floor * np.sqrt(b.size)))
 
 
-def make_solver(solid, niu, force, modes, rho_bc, u_bc, conv="reference", dtype="f32", sparse=False):
+def make_solver(solid, niu, force, modes, rho_bc, u_bc, conv="reference", dtype="f32", sparse=False,
+                band_rows=None):
     from taichi_lbm3d_amd import LB3D_Solver_Single_Phase
     nx, ny, nz = solid.shape
     s = LB3D_Solver_Single_Phase(nx, ny, nz, sparse_storage=sparse, convention=conv, dtype=dtype)
+    if band_rows is not None:
+        s.band_rows = band_rows
     s.solid.from_numpy(solid)
     s.niu = niu
     s.set_force(force)
@@ -79,15 +82,18 @@ def case_geometry(name):
 
 
 @pytest.mark.parametrize("name", sorted(CASES))
-def test_compaction_bit_exact(gpu, name):
+@pytest.mark.parametrize("band_rows", [0, 3])
+def test_compaction_bit_exact(gpu, name, band_rows):
     """Fluid list, face records and the neighbour ids the step kernel computes
-    from the row table equal the brute-force restatement bit for bit."""
+    from the row table equal the brute-force restatement bit for bit, in C
+    order and in bands of 3 rows."""
     import torch
     from taichi_lbm3d_amd import _lib
     solid = case_geometry(name)
     modes = CASES[name][4]
-    s = make_solver(solid, CASES[name][2], CASES[name][3], modes, CASES[name][5], CASES[name][6])
-    cells, d2c = oc.compact(solid)
+    s = make_solver(solid, CASES[name][2], CASES[name][3], modes, CASES[name][5], CASES[name][6],
+                    band_rows=band_rows)
+    cells, d2c = oc.compact(solid, band_rows)
     assert s.n_fluid == cells.size
     np.testing.assert_array_equal(s._dev["cells"][: cells.size].cpu().numpy(), cells)
     np.testing.assert_array_equal(s.neighbor_table(), oc.neighbors(solid, cells, d2c))
@@ -126,12 +132,14 @@ def test_compaction_pack32_fixture(gpu):
 
 
 def test_compaction_cavity_fixture(gpu):
-    """The reference's 50^3 cavity mask: 112,896 fluid cells (Single_phase/geo_cavity.dat)."""
+    """The reference's 50^3 cavity mask: 112,896 fluid cells (Single_phase/geo_cavity.dat),
+    the list in C order (the checksums are of the C-order list)."""
     import hashlib
     g = np.load(os.path.join(os.path.dirname(__file__), "golden", "cavity_compaction.npz"))
     from taichi_lbm3d_amd.geometry import cavity
     solid = cavity(50)
-    s = make_solver(solid, 0.1, (0, 0, 0), (0, 2, 0, 0, 0, 0), (1,) * 6, [(0, 0, 0), (0, 0, 0.1)] + [(0, 0, 0)] * 4)
+    s = make_solver(solid, 0.1, (0, 0, 0), (0, 2, 0, 0, 0, 0), (1,) * 6, [(0, 0, 0), (0, 0, 0.1)] + [(0, 0, 0)] * 4,
+                    band_rows=0)
     assert s.n_fluid == 112896
     cells = s._dev["cells"].cpu().numpy()
     assert hashlib.sha256(cells.astype("<i4").tobytes()).hexdigest() == str(g["sha_cells"])
@@ -157,6 +165,50 @@ def test_step_matches_oracle(gpu, name, conv, steps):
     assert er < 1e-6 and ev < 1e-5
     # non-fluid cells: rho = 1, v = 0 in dense storage (SP:390-392)
     assert np.all(rho[~fl] == 1.0) and np.all(v[~fl] == 0.0)
+
+
+@pytest.mark.parametrize("name", sorted(CASES))
+def test_step_band_order_matches_oracle(gpu, name):
+    """The fluid list in bands of 3 rows (every block of the step kernel spans
+    several planes and takes the row-table path): the same fields as the
+    oracle."""
+    solid = case_geometry(name)
+    args = CASES[name][2:]
+    s = make_solver(solid, *args, band_rows=3)
+    o = make_oracle(solid, *args)
+    s.run(40)
+    o.step(40)
+    fl = solid == 0
+    rho, v = s.rho.to_numpy(), s.v.to_numpy()
+    er, ev = rel_l2(rho[fl], o.rho()[fl]), rel_l2(v[fl], o.v()[fl])
+    print(f"{name} bands of 3: rel-L2 rho {er:.2e} v {ev:.2e}")
+    assert er < 1e-6 and ev < 1e-5
+
+
+@pytest.mark.parametrize("band_rows", [0, 16])
+def test_porous_band_order_matches_oracle(gpu, band_rows):
+    """A 12 x 48 x 128 random porous box, pressure-driven along x, with a force
+    along z: in bands of 16 rows 167 of the 202 blocks of 256 cells lie in one
+    plane's band and take the LDS-window path, the other 35 span two planes.
+    The fields after 60 steps equal the FP64 oracle's, and the two list orders
+    give the same fields bit for bit."""
+    rng = np.random.default_rng(11)
+    solid = (rng.random((12, 48, 128)) < 0.3).astype(np.int8)
+    args = (0.1, (0, 0, 2e-5), (1, 1, 0, 0, 0, 0), (1.002, 1.0, 1, 1, 1, 1), [(0, 0, 0)] * 6)
+    s = make_solver(solid, *args, band_rows=band_rows)
+    o = make_oracle(solid, *args)
+    s.run(60)
+    o.step(60)
+    fl = solid == 0
+    rho, v = s.rho.to_numpy(), s.v.to_numpy()
+    er, ev = rel_l2(rho[fl], o.rho()[fl]), rel_l2(v[fl], o.v()[fl])
+    print(f"porous bands of {band_rows}: rel-L2 rho {er:.2e} v {ev:.2e}")
+    assert er < 1e-6 and ev < 1e-5
+    if band_rows:
+        c = make_solver(solid, *args, band_rows=0)
+        c.run(60)
+        np.testing.assert_array_equal(c.rho.to_numpy(), rho)
+        np.testing.assert_array_equal(c.v.to_numpy(), v)
 
 
 def test_initial_state_macros(gpu):
