@@ -184,6 +184,7 @@ def main():
     solid = make_geometry(args.n, args.seed)
     s = make_solver(solid, args.dtype)
     n_fluid = s.n_fluid
+    band_rows = s.band_rows
     log(f"fluid cells {n_fluid}")
 
     ms = time_steps(s, args.steps, args.warmup)
@@ -240,6 +241,7 @@ def main():
                                f"6-14, porosity {float((solid == 0).mean()):.4f}), pressure x faces 1.0005/1.0, "
                                f"y/z periodic, niu 0.1, reference convention",
                    "fluid_cells": n_fluid, "storage": args.dtype, "schedule": schedule,
+                   "fluid_list": f"bands of {band_rows} grid rows" if 0 < band_rows < args.n else "C order",
                    "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
