@@ -62,6 +62,13 @@ constexpr int kWinWords = 384;       // LDS row window: 3 planes x (J+3) rows x 
 #ifndef LBM3D_STEP_MINB
 #define LBM3D_STEP_MINB 1
 #endif
+#ifndef LBM3D_STEP_WAVES16
+// Minimum waves per SIMD of the FP16/BF16 step kernels.  Left alone, their odd
+// step takes 74 VGPRs (6 waves); held to 7 waves it fits 72 without spilling:
+// FP16 85.3 / 86.4 us per step, FP16 with stochastic rounding 98.6 / 101.4,
+// BF16 with stochastic rounding 92.1 / 92.9 (8 waves spill 16-28 bytes and lose).
+#define LBM3D_STEP_WAVES16 7
+#endif
 #ifndef LBM3D_REVERSE_EVEN
 // Even steps walk the block range backwards, so they first read the slots the
 // preceding odd step wrote last, while those are still in the 256 MB last-level
@@ -1044,10 +1051,15 @@ __device__ __forceinline__ void step_tile(const KP &kp, T *__restrict__ g, const
   step_cell<T, ODD, FORCED>(kp, kp.geo, sl, rv, cells, face_bits, face_u, x, first);
 }
 
+// minimum waves per SIMD the step kernel of storage T is compiled for
+template <typename T> struct StepWaves { static constexpr int v = LBM3D_STEP_MINB; };
+template <> struct StepWaves<__half> { static constexpr int v = LBM3D_STEP_WAVES16; };
+template <> struct StepWaves<__hip_bfloat16> { static constexpr int v = LBM3D_STEP_WAVES16; };
+
 // One time step over the blocks lb_begin .. lb_begin + nb - 1 of the fluid
 // list.
 template <typename T, bool ODD, bool FORCED>
-__global__ void __launch_bounds__(kBlock, LBM3D_STEP_MINB)
+__global__ void __attribute__((amdgpu_flat_work_group_size(1, kBlock), amdgpu_waves_per_eu(StepWaves<T>::v)))
     k_step(const KP kp, T *__restrict__ g, const int32_t *__restrict__ cells, const uint4 *__restrict__ rows,
            const uint32_t *__restrict__ face_bits, typename St<T>::R *__restrict__ face_u, int first,
            uint32_t lb_begin, uint32_t nb) {
