@@ -280,7 +280,10 @@ int lbm3d_reduce_noalloc(void *stream, const lbm3d_params *p, const int32_t *cel
  *                            face_bits + cell_begin / 16;
  *   face_u[face_u_begin ...] sample s's pressure-face state, faces in face order.
  * In every batch call p->n_cells is the layout's total `cells` and p->nx,
- * ny, nz are not used.  Padding ids between samples are never read or written.
+ * ny, nz are not used.  Every sample's list is compacted with p->band_rows
+ * (lbm3d_compact_fill with the sample's extents and the batch's band_rows),
+ * and lbm3d_batch_desc records that order for the odd steps' row windows.
+ * Padding ids between samples are never read or written.
  */
 #define LBM3D_BATCH_ALIGN 512
 
