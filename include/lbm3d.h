@@ -15,8 +15,10 @@
  *     enqueued; nothing synchronises unless stated.
  *   - Return 0 on success, a negative LBM3D_E* code otherwise;
  *     lbm3d_last_error() then gives a message (per thread).
- *   - Fluid cells (solid value 0) are numbered 0 .. n_cells-1 in C order of
- *     (i, j, k), k fastest.
+ *   - Fluid cells (solid value 0) are numbered 0 .. n_cells-1 in the list
+ *     order that p->band_rows sets (lbm3d_compact_fill): every grid row's
+ *     cells get consecutive ids, k fastest; band_rows = 0 is C order of
+ *     (i, j, k).
  *   - Populations are stored shifted, g_q = f_q - w_q, in SoA order
  *     g[q * S + cell], q = 0..18 in the reference's direction order
  *     (SP:183-187), S = lbm3d_slot_stride(n_cells) (n_cells rounded up to a
