@@ -214,6 +214,10 @@ def test_abi_rejects_bad_arguments_without_gpu():
     assert rc == -3
     p.nx = p.ny = p.nz = 4
     p.n_cells = 10
+    p.band_rows = -1
+    rc = lib.cdll.lbm3d_compact_count(None, ctypes.byref(p), vp, vp, vp, 1 << 20)
+    assert rc == -1 and "band_rows" in lib.last_error()
+    p.band_rows = 0
     p.nx = 1
     p.bc_mode[0] = 1
     rc = lib.cdll.lbm3d_compact_count(None, ctypes.byref(p), vp, vp, vp, 1 << 20)
