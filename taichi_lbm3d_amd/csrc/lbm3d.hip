@@ -378,39 +378,28 @@ template <> struct St<float> {
     __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(v), r, (int)vo, (int)so, LBM3D_CPOL_STORE);
   }
 };
-// Stochastic rounding to a 16-bit format: add uniform random bits to the FP32
-// pattern below the target format's last place, then truncate toward zero.
-// The stored value equals the exact one in expectation, so an increment far
-// below the format's spacing is kept on average instead of always rounding
-// away.  rnd: 16 random bits (in bits 0-15).
+// Stochastic rounding to a 16-bit format with gfx950's conversions
+// (v_cvt_sr_bf16_f32, v_cvt_sr_f16_f32): uniform random bits are added to the
+// FP32 pattern below the target format's last place and the result is
+// truncated, so the stored value equals the exact one in expectation and an
+// increment far below the format's spacing is kept on average instead of
+// always rounding away.  The instructions take the dither from the top of
+// their 32-bit random word (BF16: its high 16 bits; FP16: its top 13 bits for
+// normal values, more below), so the 16 random bits rnd (bits 0-15) go to bits
+// 16-31.  Inf and NaN pass through (NaN as the quiet NaN).
+typedef __bf16 lbm3d_bf16x2 __attribute__((ext_vector_type(2)));
+typedef _Float16 lbm3d_f16x2 __attribute__((ext_vector_type(2)));
+
 __device__ __forceinline__ unsigned short bf16_sr(float v, uint32_t rnd) {
-  const uint32_t bits = __float_as_uint(v);
-  if ((bits & 0x7F800000u) == 0x7F800000u) return (unsigned short)(bits >> 16);  // inf / nan
-  return (unsigned short)((bits + rnd) >> 16);
+  lbm3d_bf16x2 z = {};
+  z = __builtin_amdgcn_cvt_sr_bf16_f32(z, v, rnd << 16, false);
+  return __builtin_bit_cast(unsigned short, z[0]);
 }
 
 __device__ __forceinline__ unsigned short half_sr(float v, uint32_t rnd) {
-  const uint32_t bits = __float_as_uint(v);
-  const uint32_t ex = (bits >> 23) & 0xFFu;
-  if (ex == 0xFFu) return __half_as_ushort(__float2half_rn(v));  // inf / nan
-  // float mantissa bits below the half's last place: 13 for half normals (ex >= 113),
-  // 126 - ex for half subnormals (last place 2^-24)
-  const uint32_t drop = ex >= 113u ? 13u : 126u - ex;
-  if (drop >= 24u) {  // |v| < 2^-24: 0 or the smallest subnormal, with probability |v| / 2^-24
-    const float p = fabsf(v) * 0x1p24f;
-    const unsigned short sign = (unsigned short)((bits >> 16) & 0x8000u);
-    return (float)rnd * 0x1p-16f < p ? (unsigned short)(sign | 1u) : sign;
-  }
-  // dither below the last place, then clear those bits: the result is a half
-  // value (exact under the conversion), one grid step up with probability equal
-  // to the dropped fraction (to 2^-16 of a step: the 16 random bits go to the
-  // top of a wider dropped field).  The bits are cleared rather than left to a
-  // toward-zero conversion because a carry into the next binade of the
-  // subnormal range keeps the same 2^-24 grid there.
-  const uint32_t mask = (1u << drop) - 1u;
-  const uint32_t dither = drop > 16u ? rnd << (drop - 16u) : rnd;
-  const uint32_t d = (bits + (dither & mask)) & ~mask;
-  return __half_as_ushort(__float2half_rn(__uint_as_float(d)));
+  lbm3d_f16x2 z = {};
+  z = __builtin_amdgcn_cvt_sr_f16_f32(z, v, rnd << 16, false);
+  return __builtin_bit_cast(unsigned short, z[0]);
 }
 
 template <> struct St<__half> {
