@@ -41,7 +41,7 @@
 extern "C" {
 #endif
 
-#define LBM3D_ABI_VERSION 13
+#define LBM3D_ABI_VERSION 14
 
 /* storage types for populations; F32, F16 and BF16 compute in FP32, F64 in FP64 */
 #define LBM3D_F32 0
@@ -231,6 +231,24 @@ int lbm3d_macros(void *stream, const lbm3d_params *p, int dtype, const void *g, 
 int lbm3d_scatter_dense(void *stream, const lbm3d_params *p, const int32_t *cells,
                         const float *rho, const float *v, int64_t v_stride, int32_t ox, int32_t oy,
                         int32_t oz, float *rho_dense, float *v_dense);
+
+/* No reference counterpart (SP stores FP32): the rounding the step kernels
+ * apply when they store a population in 16 bits, on n FP32 values in[] ->
+ * out[] (the 16-bit patterns), dtype LBM3D_F16 or LBM3D_BF16.  stochastic 0:
+ * round to nearest even.  Otherwise the rounding of
+ * LBM3D_FLAG_STOCHASTIC_ROUNDING with the 16 random bits rnd[i] & 0xFFFF: BF16
+ * adds them below its last place and truncates; FP16 adds the top 13 of them
+ * below its last place for normal results (their top 13 + j for results in its
+ * subnormal range, j the extra dropped bits, zero-filled below bit 16) and
+ * truncates, and below 2^-24 stores 0 or +-2^-24 with the matching
+ * probability.  Each result is one of the two values around in[i], the upper
+ * with probability equal to the dropped fraction: over all 65,536 values of
+ * the random bits the results average to in[i] exactly for BF16 and for FP16
+ * normal results, and for FP16 results below 2^-14 (its subnormal range) to
+ * in[i] less 1e-5 to 1e-4 of a step (gfx950's carry there comes up slightly
+ * short).  For tests and for converting fields to the storage formats. */
+int lbm3d_round16(void *stream, int dtype, int stochastic, const float *in, const uint32_t *rnd,
+                  uint16_t *out, int64_t n);
 
 /* FP64 reductions over the compact fields (cal_max_v, SP:399-402, plus the
  * flux and density sums used for permeability and convergence):

@@ -15,7 +15,7 @@ LIB_PATH = os.path.join(_HERE, "liblbm3d.so")
 SOURCES = [os.path.join(_HERE, "csrc", name) for name in ("lbm3d.hip", "conv3d.hip", "geo_io.hip")]
 INCLUDE = os.path.join(_ROOT, "include")
 
-ABI_VERSION = 13
+ABI_VERSION = 14
 F32, F16, BF16, F64 = 0, 1, 2, 3
 FLAG_STOCHASTIC_ROUNDING = 1
 FLAG_TILES = 2
@@ -121,6 +121,7 @@ SIGNATURES = {
     "lbm3d_scatter_dense": (_int, [_vp, _pp, _vp, _vp, _vp, _i64, _i32, _i32, _i32, _vp, _vp]),
     "lbm3d_reduce_workspace_bytes": (_sz, [_i32]),
     "lbm3d_reduce": (_int, [_vp, _pp, _vp, _vp, _vp, _i32, _i32, _vp, _vp, _sz]),
+    "lbm3d_round16": (_int, [_vp, _i32, _i32, _vp, _vp, _vp, _i64]),
     "lbm3d_reduce64": (_int, [_vp, _pp, _vp, _vp, _vp, _i32, _i32, _vp, _vp, _sz]),
     "lbm3d_reduce_noalloc": (_int, [_vp, _pp, _vp, _vp, _vp, _i32, _i32, _vp]),
     "lbm3d_batch_layout": (_int, [_pp, ctypes.POINTER(Sample), _i32, ctypes.POINTER(BatchSizes)]),

@@ -18,6 +18,7 @@
 // mask below the cell), so no per-cell neighbour table is stored or read.
 #include <hip/hip_runtime.h>
 #include <hip/hip_fp16.h>
+#include <type_traits>
 #include <hip/hip_bf16.h>
 
 #include <algorithm>
@@ -1456,6 +1457,26 @@ __global__ void __launch_bounds__(kBlock) k_neighbors(const KP kp, const int32_t
   for (int q = 0; q < 18; ++q) nbr[(size_t)q * N + x] = nb[q];
 }
 
+// lbm3d_round16: the step kernels' 16-bit store rounding applied to n values
+template <typename T>
+__global__ void __launch_bounds__(kBlock) k_round16(int stochastic, const float *__restrict__ in,
+                                                    const uint32_t *__restrict__ rnd, uint16_t *__restrict__ out,
+                                                    uint32_t n) {
+  const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n) return;
+  const float v = in[i];
+  if constexpr (std::is_same<T, __half>::value) {
+    out[i] = stochastic ? half_sr(v, rnd[i] & 0xFFFFu) : __half_as_ushort(__float2half_rn(v));
+  } else {
+    if (stochastic) {
+      out[i] = bf16_sr(v, rnd[i] & 0xFFFFu);
+    } else {
+      const __hip_bfloat16 h = __float2bfloat16(v);
+      out[i] = *reinterpret_cast<const unsigned short *>(&h);
+    }
+  }
+}
+
 __global__ void __launch_bounds__(kBlock) k_scatter(const KP kp, const int32_t *__restrict__ cells,
                                                     const float *__restrict__ rho, const float *__restrict__ v,
                                                     int64_t v_stride, int32_t ox, int32_t oy, int32_t oz,
@@ -2114,6 +2135,20 @@ int lbm3d_scatter_dense(void *stream, const lbm3d_params *p, const int32_t *cell
   const unsigned grid = (unsigned)((p->n_cells + kBlock - 1) / kBlock);
   k_scatter<<<grid, kBlock, 0, (hipStream_t)stream>>>(kp, cells, rho, v, v_stride, ox, oy, oz, rho_dense, v_dense);
   return check_launch("lbm3d_scatter_dense");
+}
+
+int lbm3d_round16(void *stream, int dtype, int stochastic, const float *in, const uint32_t *rnd, uint16_t *out,
+                  int64_t n) {
+  if (dtype != LBM3D_F16 && dtype != LBM3D_BF16) return fail(LBM3D_EINVAL, "%s", "lbm3d_round16: dtype must be F16 or BF16");
+  if (n < 0 || n > INT32_MAX) return fail(LBM3D_EINVAL, "%s", "lbm3d_round16: n out of range");
+  if (n == 0) return LBM3D_OK;
+  if (!in || !out || (stochastic && !rnd)) return fail(LBM3D_EINVAL, "%s", "NULL argument to lbm3d_round16");
+  const unsigned grid = (unsigned)((n + kBlock - 1) / kBlock);
+  if (dtype == LBM3D_F16)
+    k_round16<__half><<<grid, kBlock, 0, (hipStream_t)stream>>>(stochastic, in, rnd, out, (uint32_t)n);
+  else
+    k_round16<__hip_bfloat16><<<grid, kBlock, 0, (hipStream_t)stream>>>(stochastic, in, rnd, out, (uint32_t)n);
+  return check_launch("lbm3d_round16");
 }
 
 size_t lbm3d_reduce_workspace_bytes(int32_t n_cells) {

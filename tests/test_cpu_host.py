@@ -218,6 +218,11 @@ def test_abi_rejects_bad_arguments_without_gpu():
     rc = lib.cdll.lbm3d_compact_count(None, ctypes.byref(p), vp, vp, vp, 1 << 20)
     assert rc == -1 and "band_rows" in lib.last_error()
     p.band_rows = 0
+    rc = lib.cdll.lbm3d_round16(None, 0, 0, vp, vp, vp, 8)
+    assert rc == -1 and "dtype" in lib.last_error()
+    rc = lib.cdll.lbm3d_round16(None, 1, 1, vp, None, vp, 8)
+    assert rc == -1 and "NULL" in lib.last_error()
+    assert lib.cdll.lbm3d_round16(None, 2, 0, None, None, None, 0) == 0
     p.nx = 1
     p.bc_mode[0] = 1
     rc = lib.cdll.lbm3d_compact_count(None, ctypes.byref(p), vp, vp, vp, 1 << 20)
