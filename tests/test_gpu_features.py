@@ -235,3 +235,31 @@ def test_tile_descriptors_change_nothing(gpu, shape, bc):
         out.append(s._dev["g"].cpu())
         assert (s._params.flags & 2) == (2 if tiles else 0)
     assert torch.equal(out[0], out[1])
+
+
+def test_band_order_full_size_bit_exact(gpu):
+    """The benchmark workload at full size (256^3 pack, pressure x faces, niu
+    0.1): the fluid list in the default bands of 32 rows and in C order give
+    the same fields bit for bit after 201 steps (both AA parities and the
+    pressure faces' carried velocity), so the order moves populations in
+    memory and changes nothing else."""
+    import torch
+    from taichi_lbm3d_amd import LB3D_Solver_Single_Phase
+    from taichi_lbm3d_amd.geometry import sphere_pack
+    solid = sphere_pack((256,) * 3, porosity=0.35, rmin=6.0, rmax=14.0, seed=0, periodic=(False, True, True))
+    fields = []
+    for band in (32, 0):
+        s = LB3D_Solver_Single_Phase(256, 256, 256, sparse_storage=True)
+        s.band_rows = band
+        s.solid.from_numpy(solid)
+        s.niu = 0.1
+        s.set_bc_rho_x0(1.0005)
+        s.set_bc_rho_x1(1.0)
+        s.init_simulation()
+        s.run(201)
+        fields.append((s.rho.to_numpy(), s.v.to_numpy()))
+        del s
+        torch.cuda.empty_cache()
+    np.testing.assert_array_equal(fields[0][0], fields[1][0])
+    np.testing.assert_array_equal(fields[0][1], fields[1][1])
+    assert np.abs(fields[0][1]).max() > 0
