@@ -171,11 +171,15 @@ int lbm3d_build_rows(void *stream, const lbm3d_params *p, const int8_t *solid, c
 int lbm3d_build_rows_at(void *stream, const lbm3d_params *p, const int8_t *solid, const int32_t *d2c,
                         int32_t id_offset, lbm3d_row_word *rows);
 
-/* Tile descriptors (LBM3D_FLAG_TILES) from the fluid list of pass 2, one per
- * block of cells: {i, j, J, 0} when the block's cells lie in rows
- * (i, j .. j+J) of one plane and their row window fits the odd step's LDS
- * copy, else {i, j, 0xFFFFFFFF, 0}.  No reference counterpart: an index of the
- * row table for the odd step. */
+/* Tile descriptors (LBM3D_FLAG_TILES) from the fluid list of pass 2 (in the
+ * order of p->band_rows), one per block of cells: {i, j, J, 0} when the
+ * block's cells lie in rows (i, j .. j+J) of one plane's segment of the list;
+ * {i, j, JA | JB << 8 | 0x10000, iB | jB << 16} when they lie in rows
+ * (i, j .. j+JA) and then in the next segment's rows (iB, jB .. jB+JB), the
+ * next plane of the band or the first plane of the next band; in both cases
+ * only when their row window fits the odd step's LDS copy.  Otherwise
+ * {i, j, 0xFFFFFFFF, 0}.  No reference counterpart: an index of the row table
+ * for the odd step. */
 int lbm3d_build_tiles(void *stream, const lbm3d_params *p, const int32_t *cells, lbm3d_row_word *tiles);
 
 /* face_bits: per fluid cell, bit 2*(x%16) of word x/16 set when the cell

@@ -941,27 +941,59 @@ __device__ __forceinline__ RowView row_window(const Geo &gm, const int32_t *__re
   return RowView{&rb, nullptr, 0u, 0, 0, 0u, -1, -1, 0};
 }
 
-// Tile descriptors (LBM3D_FLAG_TILES): per block of kBlock cells, {i0, j0, J,
-// 0} when the block's cells lie in rows (i0, j0 .. j0+J) of one plane, its row
-// window fits kWinWords and the middle plane's (J+1) W words fit one wave;
-// else {i0, j0, 0xFFFFFFFF, 0}.  The odd step then stages the window straight
-// from the descriptor and finds every cell's place from the window's words,
-// without reading the cell list.
+// Tile descriptors (LBM3D_FLAG_TILES), one per block of kBlock cells.  The
+// block's cells lie in
+//   one plane's segment of the list: {i0, j0, J, 0}, rows (i0, j0 .. j0+J);
+//   two consecutive segments: {i0, j0, JA | JB << 8 | kTwoSeg, iB | jB << 16},
+//     rows (i0, j0 .. j0+JA) then (iB, jB .. jB+JB), the second segment being
+//     the plane after i0 in the same band or the first plane of the next band;
+//   else {i0, j0, kNoTile, 0}: the odd step reads the cell list and the table.
+// A descriptor's rows, with the rows and planes around them, fit kWinWords
+// words of LDS and its middle rows' words fit one wave (at most 64).  The odd
+// step stages that window straight from the descriptor and finds every cell's
+// place from the window's words, without reading the cell list.
 constexpr uint32_t kNoTile = 0xFFFFFFFFu;
+constexpr uint32_t kTwoSeg = 1u << 16;
 
-__device__ __forceinline__ RowView tile_window(const Geo &gm, const uint4 *__restrict__ rows, const Rows &rb,
-                                               const uint4 td, uint4 *win) {
-  const int i0 = (int)td.x, j0 = (int)td.y;
-  const uint32_t JW = td.z + 3u, per_plane = JW * gm.W, cnt = 3u * per_plane;
-  for (uint32_t e = threadIdx.x; e < cnt; e += kBlock) {
-    const uint32_t di = e / per_plane, r = e - di * per_plane;
+// A descriptor's segments: segment A at win[0 .. offB), B (when nB > 0) after it;
+// each is 3 planes x JW rows x W words, its middle rows' words at e (nA / nB of them).
+struct TileSegs {
+  int iA, jA, iB, jB;
+  uint32_t JWA, JWB, nA, nB, offB;
+};
+
+__device__ __forceinline__ TileSegs tile_segs(const Geo &gm, const uint4 td) {
+  const bool two = (td.z & kTwoSeg) != 0u;
+  const uint32_t JA = two ? (td.z & 0xFFu) : td.z, JB = (td.z >> 8) & 0xFFu;
+  TileSegs t;
+  t.iA = (int)td.x;
+  t.jA = (int)td.y;
+  t.iB = (int)(td.w & 0xFFFFu);
+  t.jB = (int)(td.w >> 16);
+  t.JWA = JA + 3u;
+  t.nA = (JA + 1u) * gm.W;
+  t.JWB = two ? JB + 3u : 0u;
+  t.nB = two ? (JB + 1u) * gm.W : 0u;
+  t.offB = 3u * t.JWA * gm.W;
+  return t;
+}
+
+// Stage rows i-1 .. i+1 x j-1 .. j+JW-2 of each segment (periodic wrap) in
+// LDS.  Every thread of the workgroup must call it (it synchronises).
+template <int B = kBlock>
+__device__ __forceinline__ void tile_stage(const Geo &gm, const uint4 *__restrict__ rows, const TileSegs &t,
+                                           uint4 *win) {
+  const uint32_t cnt = t.offB + 3u * t.JWB * gm.W;
+  for (uint32_t e = threadIdx.x; e < cnt; e += B) {
+    const bool b = e >= t.offB;
+    const uint32_t el = b ? e - t.offB : e, per_plane = (b ? t.JWB : t.JWA) * gm.W;
+    const uint32_t di = el / per_plane, r = el - di * per_plane;
     const uint32_t dj = r / gm.W, w = r - dj * gm.W;
-    const uint32_t row =
-        (uint32_t)wrap(i0 + (int)di - 1, gm.nx) * (uint32_t)gm.ny + (uint32_t)wrap(j0 + (int)dj - 1, gm.ny);
+    const uint32_t row = (uint32_t)wrap((b ? t.iB : t.iA) + (int)di - 1, gm.nx) * (uint32_t)gm.ny +
+                         (uint32_t)wrap((b ? t.jB : t.jA) + (int)dj - 1, gm.ny);
     win[e] = rows[row * gm.W + w];
   }
   __syncthreads();
-  return RowView{&rb, win, JW, i0, j0, ((uint32_t)i0 * (uint32_t)gm.ny + (uint32_t)j0) * (uint32_t)gm.nz, -1, -1, 0};
 }
 
 // position of the r-th (from 0) set bit of m (r < popcount(m))
@@ -978,18 +1010,20 @@ __device__ __forceinline__ uint32_t select_bit(uint32_t m, uint32_t r) {
   return pos + ((m & 1u) <= r ? 1u : 0u);
 }
 
-// Row (jrel, from j0) and k of cell x of the window's middle plane, from its
-// (J+1) W <= 64 words: lane l holds word l; an exclusive suffix minimum gives
-// empty words the first id of the next word with a fluid cell, and a binary
-// search over the lanes finds the last word whose first id is <= x.  Every
-// lane of the wave must call it.
-__device__ __forceinline__ void tile_place(const Geo &gm, const uint4 *win, uint32_t JW, uint32_t J, uint32_t x,
-                                           int &jrel, int &k) {
-  const uint32_t W = gm.W, nw = (J + 1u) * W, e0 = (JW + 1u) * W;
+// Row (jrel, from its segment's first row) and k of cell x, and the window of
+// its segment, from the segments' middle-row words (nA + nB <= 64, in list
+// order): lane l holds word l; an exclusive suffix minimum gives empty words
+// the first id of the next word with a fluid cell, and a binary search over
+// the lanes finds the last word whose first id is <= x.  Every lane of the
+// wave must call it.
+__device__ __forceinline__ void tile_place(const Geo &gm, const uint4 *win, const TileSegs &t, uint32_t x,
+                                           RowView &rv) {
+  const uint32_t W = gm.W, nw = t.nA + t.nB;
+  const uint32_t eA = (t.JWA + 1u) * W, eB = t.offB + (t.JWB + 1u) * W;
   const uint32_t lane = threadIdx.x & 63u;
   uint32_t key = 0x7FFFFFFFu;
   if (lane < nw) {
-    const uint4 wd = win[e0 + lane];
+    const uint4 wd = win[lane < t.nA ? eA + lane : eB + (lane - t.nA)];
     if (wd.x) key = wd.y;
   }
 #pragma unroll
@@ -1004,9 +1038,15 @@ __device__ __forceinline__ void tile_place(const Geo &gm, const uint4 *win, uint
     const uint32_t kc = (uint32_t)__shfl((int)key, (int)(cand < 64u ? cand : 63u));
     if (cand < nw && kc <= x) pos = cand;
   }
-  const uint4 hw = win[e0 + pos];
-  k = (int)(32u * (pos % W) + select_bit(hw.x, x - hw.y));
-  jrel = (int)(pos / W);
+  const bool segb = pos >= t.nA;
+  const uint32_t p = segb ? pos - t.nA : pos;
+  const uint4 hw = win[segb ? eB + p : eA + p];
+  rv.k = (int)(32u * (p % W) + select_bit(hw.x, x - hw.y));
+  rv.jrel = (int)(p / W);
+  rv.win = segb ? win + t.offB : win;
+  rv.JW = segb ? t.JWB : t.JWA;
+  rv.i0 = segb ? t.iB : t.iA;
+  rv.j0 = segb ? t.jB : t.jA;
 }
 
 // One tile (kBlock consecutive fluid cells from lb * kBlock) of a time step.
@@ -1027,8 +1067,9 @@ __device__ __forceinline__ void step_tile(const KP &kp, T *__restrict__ g, const
     if (kp.p.flags & LBM3D_FLAG_TILES) {
       const uint4 td = rows[kp.geo.n_row_words + lb];  // this block's descriptor
       if (td.z != kNoTile) {
-        rv = tile_window(kp.geo, rows, rb, td, win);
-        tile_place(kp.geo, win, rv.JW, td.z, x, rv.jrel, rv.k);
+        const TileSegs ts = tile_segs(kp.geo, td);
+        tile_stage(kp.geo, rows, ts, win);
+        tile_place(kp.geo, win, ts, x, rv);
       } else {
         rv.cell = cells[x < n ? x : n - 1];  // table path
       }
@@ -1205,8 +1246,9 @@ __global__ void __launch_bounds__(B)
       if (kp.p.flags & LBM3D_FLAG_TILES) {  // as k_step: the window from this block's descriptor
         const uint4 td = rows[d.tiles_begin + (x0 - d.cell_begin) / kBlock];
         if (td.z != kNoTile) {
-          rv = tile_window(d.geo, rows + d.rows_begin, rb, td, win);
-          tile_place(d.geo, win, rv.JW, td.z, x, rv.jrel, rv.k);
+          const TileSegs ts = tile_segs(d.geo, td);
+          tile_stage(d.geo, rows + d.rows_begin, ts, win);
+          tile_place(d.geo, win, ts, x, rv);
         } else {
           rv.cell = cells[x < d.cell_end ? x : d.cell_end - 1];  // table path
         }
@@ -1408,12 +1450,33 @@ __global__ void __launch_bounds__(kBlock) k_build_tiles(const Geo gm, const int3
   int i0, j0, k0, i1, j1, k1;
   coords(gm, (uint32_t)cells[x0], i0, j0, k0);
   coords(gm, (uint32_t)cells[xl], i1, j1, k1);
-  const uint32_t J = (uint32_t)(j1 - j0);
-  // the block lies in rows j0 .. j1 of one plane when its first and last cells
-  // are in the same plane and the same band of the list order
-  const bool ok = i0 == i1 && (uint32_t)j0 / gm.band == (uint32_t)j1 / gm.band &&
-                  3u * (J + 3u) * gm.W <= (uint32_t)kWinWords && (J + 1u) * gm.W <= 64u;
-  tiles[t] = make_uint4((uint32_t)i0, (uint32_t)j0, ok ? J : kNoTile, 0u);
+  const uint32_t bandA = (uint32_t)j0 / gm.band, bandL = (uint32_t)j1 / gm.band;
+  // one segment: the block's first and last cells are in the same plane and
+  // the same band of the list order, so every cell between lies in rows j0 .. j1
+  if (i0 == i1 && bandA == bandL) {
+    const uint32_t J = (uint32_t)(j1 - j0);
+    const bool ok = 3u * (J + 3u) * gm.W <= (uint32_t)kWinWords && (J + 1u) * gm.W <= 64u;
+    tiles[t] = make_uint4((uint32_t)i0, (uint32_t)j0, ok ? J : kNoTile, 0u);
+    return;
+  }
+  // two segments: the last cell lies in the segment after the first cell's
+  // (the next plane of the band, or the first plane of the next band), so the
+  // block is the end of the first segment, rows j0 .. its band's last row,
+  // and the start of the second, rows jB .. j1
+  const bool next_band = i0 + 1 == gm.nx;
+  const int iB = next_band ? 0 : i0 + 1;
+  const uint32_t bandB = next_band ? bandA + 1u : bandA;
+  bool ok = i1 == iB && bandL == bandB && gm.nx <= 0xFFFF && gm.ny <= 0xFFFF;
+  uint32_t JA = 0u, JB = 0u, jB = 0u;
+  if (ok) {
+    const uint32_t endA = min((bandA + 1u) * gm.band, (uint32_t)gm.ny) - 1u;
+    jB = bandB * gm.band;
+    JA = endA - (uint32_t)j0;
+    JB = (uint32_t)j1 - jB;
+    ok = 3u * (JA + 3u + JB + 3u) * gm.W <= (uint32_t)kWinWords && (JA + 1u + JB + 1u) * gm.W <= 64u;
+  }
+  tiles[t] = ok ? make_uint4((uint32_t)i0, (uint32_t)j0, JA | (JB << 8) | kTwoSeg, (uint32_t)iB | (jB << 16))
+                : make_uint4((uint32_t)i0, (uint32_t)j0, kNoTile, 0u);
 }
 
 // face_bits: 16 cells per word, 2 bits per cell
