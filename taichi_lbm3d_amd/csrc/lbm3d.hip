@@ -718,16 +718,18 @@ struct CellCtx {
   bool have_coords;
 };
 
-// Row words for the odd-step neighbour lookup: the table in memory, or the
-// workgroup's LDS window when win != nullptr.
+// Row words for the odd-step neighbour lookup: the table in memory, or, when
+// win != nullptr, the LDS window of the cell's plane segment (3 planes x JW
+// rows x W words around rows (i0, j0 ..); a tile that spans two segments
+// stages one window for each, and each lane points at its own).
 struct RowView {
   const Rows *rb;
   const uint4 *win;
   uint32_t JW;
   int i0, j0;
-  uint32_t c0;  // dense index of cell (i0, j0, 0)
+  uint32_t c0;  // dense index of cell (i0, j0, 0) (row_window's one-segment path)
   int32_t cell; // >= 0: the cell's dense index, loaded ahead (else read from cells[])
-  int jrel, k;  // cell >= 0 and jrel >= 0: the cell's row in the window and k, found without cells[]
+  int jrel, k;  // jrel >= 0: the cell's row (from j0) in the window and its k, found without cells[]
 };
 
 // Odd step: byte offsets off[q] of the addresses A_q within q's slot pair, and
