@@ -248,3 +248,27 @@ def test_get_max_v(gpu):
     o.step(12)
     assert abs(s.get_max_v() - o.get_max_v()) <= 1e-6 * o.get_max_v()
     assert s.max_v[None] == s.get_max_v()
+
+
+@pytest.mark.parametrize("band_rows", [3, 16])
+def test_f64_band_order_matches_fp64_oracle(gpu, band_rows):
+    """FP64 storage and arithmetic on the 12 x 48 x 128 porous box with the list
+    in bands of rows (in bands of 16, 35 of the 202 blocks span two planes'
+    segments and use two-segment windows; in bands of 3 most blocks span
+    several segments): the same algorithm as the FP64 oracle to round-off."""
+    rng = np.random.default_rng(11)
+    solid = (rng.random((12, 48, 128)) < 0.3).astype(np.int8)
+    args = (0.1, (0, 0, 2e-5), (1, 1, 0, 0, 0, 0), (1.002, 1.0, 1, 1, 1, 1), [(0, 0, 0)] * 6)
+    s = make_solver(solid, *args, band_rows=band_rows, dtype="f64")
+    o = make_oracle(solid, *args)
+    s.run(41)
+    o.step(41)
+    rho, v = s.compact_macros(native=True)
+    cells = s._dev["cells"].cpu().numpy()
+    n = s.n_fluid
+    rho = rho.cpu().numpy()[:n]
+    v = v.cpu().numpy()[: 3 * n].reshape(3, n).T
+    orho = o.rho().reshape(-1)[cells]
+    ov = o.v().reshape(-1, 3)[cells]
+    assert np.abs(rho - orho).max() < 1e-13
+    assert rel_l2(v, ov, floor=1e-12) < 1e-10
